@@ -1,0 +1,226 @@
+/*
+ * sw2d.h -- C ABI of the MI355X-native SW2D time-stepping core.
+ *
+ * The reference (LabHidro/SW2D-GPU, one CUDA translation unit) has no
+ * library interface: its path is reached through the `SW2D-GPU` executable,
+ * the case files it reads, and its kernel signatures.  Each entry point
+ * below names the reference interface it replaces
+ * (L:n = SW2D-GPU_linux/src/SW2D-GPU.cu line n).
+ *
+ * Conventions: every call returns 0 on success or a negative SW2D_E_* code;
+ * sw2d_last_error() gives the message of the last failure on the calling
+ * thread.  The library owns device memory; callers own host buffers.  Each
+ * context has its own HIP stream; calls on one context are not thread-safe,
+ * distinct contexts are independent.
+ *
+ * Grid layout seen through this ABI (row-major, double precision):
+ *   cells   rows x cols          i = DEM row (north->south), j = column
+ *   U faces (rows+1) x cols      U(i,j) between cells (i-1,j) and (i,j)
+ *   V faces rows x (cols+1)      V(i,j) between cells (i,j-1) and (i,j)
+ */
+#ifndef SW2D_H
+#define SW2D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SW2D_OK 0
+#define SW2D_E_INVALID -1   /* bad argument or inconsistent sizes       */
+#define SW2D_E_DEVICE -2    /* HIP runtime failure / no usable GPU      */
+#define SW2D_E_IO -3        /* missing or malformed case file           */
+#define SW2D_E_MEMORY -4    /* allocation failed or over the budget     */
+#define SW2D_E_STATE -5     /* call not valid in the current state      */
+#define SW2D_E_NONFINITE -6 /* a non-finite value reached the state     */
+
+/* open (free-outflow) edges; closed walls otherwise */
+#define SW2D_EDGE_NORTH 1 /* U faces of row 0    */
+#define SW2D_EDGE_SOUTH 2 /* U faces of row rows */
+#define SW2D_EDGE_WEST 4  /* V faces of column 0 */
+#define SW2D_EDGE_EAST 8  /* V faces of column cols */
+
+#define SW2D_FP64 0 /* all state and arithmetic in double (parity path)     */
+#define SW2D_FP32 1 /* float state, bed relative to a per-tile datum, FP64
+                       threshold tests and volume accumulators              */
+
+#define SW2D_FORCING_REFERENCE 0 /* interpolation formula of L:538/L:552   */
+#define SW2D_FORCING_STEP 1      /* piecewise constant table[int(t/dT)]    */
+
+typedef struct sw2d_ctx sw2d_ctx;
+
+/* Grid and statics.  Replaces the host arrays h_baseo / h_initial_condition
+ * and the launch set-up of L:815-834, L:1022-1033, L:1163-1241. */
+typedef struct {
+  int rows, cols;
+  double dx, dy;           /* cellsize (L:826, L:1236-1237)                */
+  double nodata;           /* NODATA_value: z == nodata marks an inactive
+                              cell (init_inf L:39-44)                      */
+  const double *z;         /* rows*cols bed elevation                      */
+  const double *h0;        /* rows*cols initial depth                      */
+  const double *n_cell;    /* optional rows*cols Manning raster, or NULL   */
+} sw2d_grid;
+
+/* Scalars of setup.dat (L:788-809, L:1130) plus the extensions. */
+typedef struct {
+  double dt;               /* half step; one iteration advances 2*dt       */
+  double g;                /* 9.80 in the reference (L:809)                */
+  double th;               /* dry threshold, 1e-4 (L:1130)                 */
+  double manning;          /* scalar n (L:799), used without a raster      */
+  double INT, INF, LWL, EV_WL_min; /* runoff fractions (L:627-634)         */
+  int evaporation_on;
+  double cfl;              /* 0: fixed step 2*dt; >0 Courant number with
+                              the step capped at 2*dt                      */
+  int edge_open;           /* SW2D_EDGE_* bitmask                          */
+  int forcing_mode;        /* SW2D_FORCING_*                               */
+  int precision;           /* SW2D_FP64 or SW2D_FP32                       */
+  double t0;               /* start time (L:1082)                          */
+  double t_end;            /* stop when t + dt > t_end (L:1256); <= 0: none */
+} sw2d_params;
+
+/* Fills p with the reference defaults (g = 9.80, th = 1e-4, FP64, ...). */
+void sw2d_params_default(sw2d_params *p);
+
+/* Allocates the device state and uploads the statics; replaces the
+ * cudaMalloc/cudaMemcpy block L:1163-1215 and init kernels init_inf L:29,
+ * initiald L:52. */
+int sw2d_create(const sw2d_grid *grid, const sw2d_params *params,
+                sw2d_ctx **out);
+void sw2d_destroy(sw2d_ctx *ctx);
+
+/* Forcing tables, kept on the device and evaluated there each half step.
+ * Replaces stream_flow L:526-555 and the per-step H2D copies of d_ql/d_rr
+ * (L:1278-1280, L:1302-1305).  table_mm: rain.dat values [mm per dtrain].
+ * block_map (optional): rain multiplier per block x block cells, row-major
+ * over ceil(rows/block) x ceil(cols/block) blocks; block must divide 64. */
+int sw2d_set_rain(sw2d_ctx *ctx, const double *table_mm, int n, double dtrain,
+                  const double *block_map, int block);
+/* q: n_rows x nst discharges [m3/s] (Q_source_sink.dat), dtoq, and each
+ * source's linear cell index computed as in L:542-545 (indices outside
+ * 0..rows*cols-1 are ignored; a later source on the same cell wins). */
+int sw2d_set_sources(sw2d_ctx *ctx, const double *q, int n_rows, int nst,
+                     double dtoq, const int64_t *cell_index);
+/* evapo_mm: evaporation per dtrain [mm] (gpu_evaporation_calc L:557-587
+ * output, see sw2d_evaporation_table); replaces evaporation_load L:589. */
+int sw2d_set_evaporation(sw2d_ctx *ctx, const double *evapo_mm, int n);
+/* Replay a fixed sequence of steps D (overrides cfl); n = 0 clears it. */
+int sw2d_set_dt_schedule(sw2d_ctx *ctx, const double *D, int n);
+/* Record the step D of every iteration into a ring of `capacity` entries
+ * (0 disables); read back with sw2d_get_dt_history. */
+int sw2d_record_dt(sw2d_ctx *ctx, int capacity);
+int sw2d_get_dt_history(sw2d_ctx *ctx, double *out, int capacity, int *n_out);
+
+/* Enqueue n iterations of the loop body L:1256-1336 (flux, continuity,
+ * treat_error, continuity, derived fields, forward) on the context stream.
+ * Iterations past t_end are skipped on the device.  Asynchronous. */
+int sw2d_step(sw2d_ctx *ctx, int64_t n_iter);
+/* Kernel timing (no reference counterpart; the reference times the whole
+ * loop with std::clock, L:1251-1252, L:1477).  After sw2d_profile_begin,
+ * each sw2d_step records HIP events on the context stream around the three
+ * launches of every iteration (up to max_iters iterations);
+ * sw2d_profile_end returns the summed milliseconds of [control, momentum,
+ * continuity] and the number of iterations timed. */
+int sw2d_profile_begin(sw2d_ctx *ctx, int max_iters);
+int sw2d_profile_end(sw2d_ctx *ctx, double ms_out[3], int64_t *n_iter);
+
+/* Wait for queued work; returns SW2D_E_NONFINITE if the state went bad. */
+int sw2d_sync(sw2d_ctx *ctx);
+
+#define SW2D_FIELD_H 0  /* rows*cols depth (inactive cells read 0)         */
+#define SW2D_FIELD_U 1  /* (rows+1)*cols unit discharge on U faces         */
+#define SW2D_FIELD_V 2  /* rows*(cols+1) unit discharge on V faces         */
+#define SW2D_FIELD_UU 3 /* rows*cols cell velocity along i (uu_vv L:459)   */
+#define SW2D_FIELD_VV 4 /* rows*cols cell velocity along j                 */
+#define SW2D_FIELD_Z 5  /* rows*cols bed as used on the device             */
+/* Copies one field to host memory in double precision (synchronises). */
+int sw2d_read(sw2d_ctx *ctx, int field, double *host_out);
+/* Overwrites the state (h, U, V) from host arrays (synchronises). */
+int sw2d_write_state(sw2d_ctx *ctx, const double *h, const double *U,
+                     const double *V);
+
+typedef struct {
+  double t;                /* model time                                   */
+  int64_t iterations;      /* completed loop iterations                    */
+  double last_D;           /* step of the last iteration                   */
+  int done;                /* t_end reached                                */
+  double volume;           /* sum of h*dx*dy over active cells [m3]        */
+  double rain_in, source_in, evap_out, outflow, clamp_gain; /* [m3]        */
+} sw2d_stats;
+int sw2d_get_stats(sw2d_ctx *ctx, sw2d_stats *out);
+
+/* Snapshot of (h, U, V, t, iteration, accumulators) in a host buffer;
+ * with buf == NULL only *size is set.  Restoring gives a bitwise
+ * identical continuation. */
+int sw2d_snapshot(sw2d_ctx *ctx, void *buf, size_t *size);
+int sw2d_restore(sw2d_ctx *ctx, const void *buf, size_t size);
+
+/* Device bytes held by a context with these dimensions and precision. */
+size_t sw2d_footprint_bytes(int rows, int cols, int precision, int has_n);
+
+const char *sw2d_last_error(void);
+
+/* Helpers restating host-side formulas of the reference. */
+/* gpu_evaporation_calc L:561-582 over n meteorological records. */
+int sw2d_evaporation_table(double albedo, const double *T, const double *Rg,
+                           int n, double dtrain, double *evapo_mm_out);
+/* stream_flow / evaporation_load interpolation (L:533-538, L:547-552). */
+double sw2d_forcing_value(double time, const double *table, int n, double dT,
+                          int mode);
+
+/* ---- reference case directories (db/<case>/input, output) ------------ */
+typedef struct sw2d_case sw2d_case;
+
+typedef struct {
+  int rows, cols;
+  double xll, yll, cellsize, nodata;
+  double tstart[4], tend[4];     /* day hour min sec                       */
+  double dt, dpout, dkout, dtoq;
+  int evaporation_on;
+  double EV_WL_min, INT, INF, LWL, manning_coef;
+  int out_velocity_x, out_velocity_y, out_elevation, out_depth,
+      out_outlet_on;
+  int nst, n_q_rows, n_rain, n_out;
+  double dtrain, albedo;
+  double time0, timmax;          /* L:1082-1087                            */
+} sw2d_case_info;
+
+/* Parses db/<case>/input/ exactly as L:779-1077 reads it. */
+int sw2d_case_load(const char *case_dir, sw2d_case **out);
+void sw2d_case_free(sw2d_case *c);
+int sw2d_case_get_info(const sw2d_case *c, sw2d_case_info *info);
+
+#define SW2D_CASE_DEM 0       /* rows*cols                                 */
+#define SW2D_CASE_IC 1        /* rows*cols                                 */
+#define SW2D_CASE_RAIN 2      /* n_rain                                    */
+#define SW2D_CASE_Q 3         /* n_q_rows*nst                              */
+#define SW2D_CASE_SRC_XY 4    /* 2*nst, divided by cellsize (L:853-854)    */
+#define SW2D_CASE_OUT_XY 5    /* 2*n_out, divided by cellsize (L:1065)     */
+#define SW2D_CASE_TEMP 6      /* n_rain                                    */
+#define SW2D_CASE_RG 7        /* n_rain                                    */
+#define SW2D_CASE_EVAPO 8     /* n_rain (derived, L:557-587)               */
+/* Copies one parsed array; *n_out receives its length. */
+int sw2d_case_get_array(const sw2d_case *c, int which, double *out,
+                        size_t cap, size_t *n_out);
+/* Linear cell index of each source as L:542-545 computes it. */
+int sw2d_case_source_cells(const sw2d_case *c, int64_t *out, int cap);
+/* Creates a context for the case with its setup.dat scalars and forcing
+ * tables; params may override precision/cfl/edges (NULL = defaults). */
+int sw2d_case_create(const sw2d_case *c, const sw2d_params *overrides,
+                     sw2d_ctx **out);
+
+/* Writes Results_<k>.vtk in the byte format of L:1364-1472. depth, vv, uu,
+ * z are rows*cols arrays in DEM order. */
+int sw2d_write_vtk(const char *path, const sw2d_case *c, const double *depth,
+                   const double *vv, const double *uu, const double *z);
+
+/* The drop-in driver: runs every case listed in <db_root>/dir.txt as the
+ * reference main() (L:728-1530) does, writing the same output files.
+ * Progress lines go to stdout.  Returns 0 or the first error. */
+int sw2d_run_db(const char *db_root, const sw2d_params *overrides);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,810 @@
+// sw2d_api.hip -- host runtime behind include/sw2d.h: device memory,
+// uploads, the per-iteration launch sequence, reads, stats and snapshots.
+//
+// Replaces the reference's main()-resident set-up and loop
+// (SW2D-GPU_linux/src/SW2D-GPU.cu L:1105-1336): 16 FP64 + 4 int arrays and
+// 9 launches + 9 device syncs + 4 H2D copies per iteration become ping-pong
+// state buffers, device-resident forcing tables and three stream-ordered
+// launches per iteration with no host synchronisation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "sw2d.h"
+#include "sw2d_launch.hpp"
+
+using sw2d::Ctrl;
+using sw2d::Grid;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                                        \
+  do {                                                                                   \
+    hipError_t e_ = (x);                                                                 \
+    if (e_ != hipSuccess) return fail(SW2D_E_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; }
+
+struct Comp {
+  double s = 0, c = 0;
+  void add(double x) {
+    double t = s + x;
+    if (std::fabs(s) >= std::fabs(x)) c += (s - t) + x;
+    else c += (x - t) + s;
+    s = t;
+  }
+  double value() const { return s + c; }
+};
+
+}  // namespace
+
+void sw2d_set_error_text(const std::string &m) { g_err = m; }
+
+struct sw2d_ctx {
+  int prec = 0;
+  size_t esz = 8;
+  int R = 0, C = 0, P = 0, TH = 0, tiles_x = 0, tiles_y = 0, n_tiles = 0;
+  sw2d_params prm{};
+  double nodata = 0;
+  double dx = 0, dy = 0;
+  hipStream_t stream = nullptr;
+  std::vector<void *> allocs;
+  void *h[2]{}, *U[2]{}, *V[2]{}, *Us = nullptr, *Vs = nullptr, *z = nullptr, *ncell = nullptr;
+  float *datum = nullptr;
+  double *rain_map = nullptr, *acc = nullptr, *partial = nullptr;
+  double *d_uu = nullptr, *d_vv = nullptr;
+  Ctrl *d_ctrl = nullptr;
+  void *d_grid = nullptr;
+  double *d_rain = nullptr, *d_evapo = nullptr, *d_q = nullptr, *d_dsched = nullptr,
+         *d_dthist = nullptr, *d_ql = nullptr;
+  int *d_src_col = nullptr;
+  long long *d_src_cell = nullptr;
+  int vol_blocks = 0;
+  int dthist_cap = 0;
+  std::vector<unsigned char> active;  // rows*cols
+  std::vector<float> datum_host;
+  int datum_cols = 0;
+  // kernel timing: 4 events per iteration (before control, after control,
+  // after momentum, after continuity)
+  std::vector<hipEvent_t> prof_ev;
+  int prof_cap = 0, prof_used = 0;
+
+  ~sw2d_ctx() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
+    for (void *p : allocs) (void)hipFree(p);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  int alloc(void **p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes ? bytes : 8);
+    if (e != hipSuccess) {
+      *p = nullptr;
+      return fail(SW2D_E_MEMORY, std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    }
+    allocs.push_back(*p);
+    return SW2D_OK;
+  }
+  void release(void *p) {
+    if (!p) return;
+    auto it = std::find(allocs.begin(), allocs.end(), p);
+    if (it != allocs.end()) allocs.erase(it);
+    (void)hipFree(p);
+  }
+
+  int fetch_ctrl(Ctrl &c) {
+    HIPCHK(hipStreamSynchronize(stream));
+    HIPCHK(hipMemcpy(&c, d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+    return SW2D_OK;
+  }
+  int store_ctrl(const Ctrl &c) {
+    HIPCHK(hipMemcpy(d_ctrl, &c, sizeof(Ctrl), hipMemcpyHostToDevice));
+    return SW2D_OK;
+  }
+  int blocks_for(long long n) const {
+    long long b = (n + sw2d::BLOCK - 1) / sw2d::BLOCK;
+    return (int)std::max(1LL, std::min(b, 4096LL));
+  }
+  int recompute_cfl() {
+    Ctrl c;
+    int rc = fetch_ctrl(c);
+    if (rc) return rc;
+    c.smax_bits = 0;
+    rc = store_ctrl(c);
+    if (rc) return rc;
+    HIPCHK(sw2d::launch_cfl_state(prec, d_grid, 0, blocks_for((long long)R * C), stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    return SW2D_OK;
+  }
+  // copy a device array with pitch P into a dense host double array
+  int read_pitched(const void *src, int rows, int cols, double *out) {
+    std::vector<unsigned char> buf((size_t)rows * P * esz);
+    HIPCHK(hipMemcpy(buf.data(), src, buf.size(), hipMemcpyDeviceToHost));
+    for (int i = 0; i < rows; i++)
+      for (int j = 0; j < cols; j++) {
+        size_t k = (size_t)i * P + j;
+        out[(size_t)i * cols + j] = prec == SW2D_FP64 ? ((const double *)buf.data())[k]
+                                                      : (double)((const float *)buf.data())[k];
+      }
+    return SW2D_OK;
+  }
+  int write_pitched(void *dst, int rows, int cols, const double *in, bool cells) {
+    std::vector<unsigned char> buf((size_t)rows * P * esz, 0);
+    for (int i = 0; i < rows; i++)
+      for (int j = 0; j < cols; j++) {
+        double v = in[(size_t)i * cols + j];
+        if (cells && !active[(size_t)i * C + j]) v = -1.0;
+        size_t k = (size_t)i * P + j;
+        if (prec == SW2D_FP64) ((double *)buf.data())[k] = v;
+        else ((float *)buf.data())[k] = (float)v;
+      }
+    HIPCHK(hipMemcpy(dst, buf.data(), buf.size(), hipMemcpyHostToDevice));
+    return SW2D_OK;
+  }
+};
+
+extern "C" {
+
+const char *sw2d_last_error(void) { return g_err.c_str(); }
+
+void sw2d_params_default(sw2d_params *p) {
+  std::memset(p, 0, sizeof(*p));
+  p->dt = 0.01;
+  p->g = 9.80;        // L:809
+  p->th = 1.0e-4;     // L:1130
+  p->manning = 0.012;
+  p->precision = SW2D_FP64;
+  p->forcing_mode = SW2D_FORCING_REFERENCE;
+}
+
+size_t sw2d_footprint_bytes(int rows, int cols, int precision, int has_n) {
+  const size_t esz = precision == SW2D_FP64 ? 8 : 4;
+  const size_t P = round_up((size_t)cols + 1, 64);
+  const int TH = precision == SW2D_FP64 ? sw2d::TILE_H_F64 : sw2d::TILE_H_F32;
+  const size_t tiles = ((size_t)(cols + 63) / 64) * (((size_t)rows + TH - 1) / TH);
+  size_t cells = (size_t)rows * P, faces = (size_t)(rows + 1) * P;
+  size_t b = esz * (2 * cells + 2 * faces + 2 * cells + faces + cells + cells);
+  if (has_n) b += esz * cells;
+  b += tiles * 2 * sw2d::ACC_N * sizeof(double);
+  b += 4 * (((size_t)rows + 63) / 64) * (((size_t)cols + 63) / 64);
+  return b + 65536;
+}
+
+double sw2d_forcing_value(double time, const double *table, int n, double dT, int mode) {
+  return sw2d::forcing_value(time, table, n, dT, mode);
+}
+
+int sw2d_evaporation_table(double albedo, const double *T, const double *Rg, int n, double dtrain,
+                           double *out) {
+  if (n < 0 || (n > 0 && (!T || !Rg || !out))) return fail(SW2D_E_INVALID, "evaporation table args");
+  for (int i = 0; i < n; i++) {
+    // gpu_evaporation_calc L:561-582 (net radiation, ITS-90 water density,
+    // Henderson-Sellers latent heat, energy-balance evaporation)
+    const double t = T[i];
+    const double Rs = (1 - albedo) * Rg[i];
+    const double pw = 999.85308 + 6.32693 * (0.01) * t - 8.523829 * (0.001) * t * t +
+                      6.943248 * (0.00001) * t * t * t - 3.821216 * (0.0000001) * t * t * t * t;
+    const double lv = (((t + 273.15) / ((t + 273.15) - 33.91)) * ((t + 273.15) / ((t + 273.15) - 33.91))) *
+                      1.91846 * 1000000;
+    double e = (Rs / (lv * pw)) * dtrain * 1000.0;
+    if (e < 0) e = 0;
+    out[i] = e;
+  }
+  return SW2D_OK;
+}
+
+int sw2d_create(const sw2d_grid *grid, const sw2d_params *params, sw2d_ctx **out) {
+  if (!grid || !params || !out) return fail(SW2D_E_INVALID, "null argument");
+  *out = nullptr;
+  const int R = grid->rows, C = grid->cols;
+  if (R <= 0 || C <= 0 || !grid->z || !grid->h0) return fail(SW2D_E_INVALID, "grid dims or arrays missing");
+  if ((long long)R * C > (1LL << 33)) return fail(SW2D_E_INVALID, "grid too large");
+  if (!(grid->dx > 0) || !(grid->dy > 0)) return fail(SW2D_E_INVALID, "cell size must be > 0");
+  if (params->precision != SW2D_FP64 && params->precision != SW2D_FP32)
+    return fail(SW2D_E_INVALID, "precision must be SW2D_FP64 or SW2D_FP32");
+  if (!(params->dt > 0)) return fail(SW2D_E_INVALID, "dt must be > 0");
+  if (params->cfl < 0) return fail(SW2D_E_INVALID, "cfl must be >= 0");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+    return fail(SW2D_E_DEVICE, "no HIP device available");
+
+  sw2d_ctx *x = new sw2d_ctx();
+  x->prec = params->precision;
+  x->esz = x->prec == SW2D_FP64 ? 8 : 4;
+  x->R = R;
+  x->C = C;
+  x->P = (int)round_up((size_t)C + 1, 64);
+  x->TH = x->prec == SW2D_FP64 ? sw2d::TILE_H_F64 : sw2d::TILE_H_F32;
+  x->tiles_x = (C + sw2d::TILE_W - 1) / sw2d::TILE_W;
+  x->tiles_y = (R + x->TH - 1) / x->TH;
+  x->n_tiles = x->tiles_x * x->tiles_y;
+  x->prm = *params;
+  x->nodata = grid->nodata;
+  x->dx = grid->dx;
+  x->dy = grid->dy;
+  auto bad = [&](int rc) {
+    delete x;
+    return rc;
+  };
+  if (hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking) != hipSuccess)
+    return bad(fail(SW2D_E_DEVICE, "hipStreamCreate failed"));
+
+  const size_t cells = (size_t)R * x->P, faces = (size_t)(R + 1) * x->P;
+  int rc = 0;
+  for (int b = 0; b < 2 && !rc; b++) {
+    rc = rc ? rc : x->alloc(&x->h[b], cells * x->esz);
+    rc = rc ? rc : x->alloc(&x->U[b], faces * x->esz);
+    rc = rc ? rc : x->alloc(&x->V[b], cells * x->esz);
+  }
+  rc = rc ? rc : x->alloc(&x->Us, faces * x->esz);
+  rc = rc ? rc : x->alloc(&x->Vs, cells * x->esz);
+  rc = rc ? rc : x->alloc(&x->z, cells * x->esz);
+  rc = rc ? rc : x->alloc((void **)&x->acc, (size_t)x->n_tiles * 2 * sw2d::ACC_N * sizeof(double));
+  x->vol_blocks = x->blocks_for((long long)R * C);
+  rc = rc ? rc : x->alloc((void **)&x->partial, (size_t)x->vol_blocks * 2 * sizeof(double));
+  rc = rc ? rc : x->alloc((void **)&x->d_ctrl, sizeof(Ctrl));
+  rc = rc ? rc : x->alloc(&x->d_grid, x->prec == SW2D_FP64 ? sizeof(Grid<double>) : sizeof(Grid<float>));
+  if (rc) return bad(rc);
+
+  // active mask (init_inf L:39-44) and statics
+  x->active.resize((size_t)R * C);
+  for (size_t k = 0; k < (size_t)R * C; k++) x->active[k] = grid->z[k] != grid->nodata;
+  std::vector<double> zdev((size_t)R * C);
+  x->datum_cols = (C + sw2d::DATUM_BLOCK - 1) / sw2d::DATUM_BLOCK;
+  const int datum_rows = (R + sw2d::DATUM_BLOCK - 1) / sw2d::DATUM_BLOCK;
+  if (x->prec == SW2D_FP32) {
+    // bed datum per 64x64 block: a multiple of 1/16 m at or below the lowest
+    // active bed, so datum differences between blocks are exact in FP32
+    x->datum_host.assign((size_t)datum_rows * x->datum_cols, 0.0f);
+    for (int bi = 0; bi < datum_rows; bi++)
+      for (int bj = 0; bj < x->datum_cols; bj++) {
+        double m = std::numeric_limits<double>::infinity();
+        for (int i = bi * 64; i < std::min(R, bi * 64 + 64); i++)
+          for (int j = bj * 64; j < std::min(C, bj * 64 + 64); j++)
+            if (x->active[(size_t)i * C + j]) m = std::min(m, grid->z[(size_t)i * C + j]);
+        x->datum_host[(size_t)bi * x->datum_cols + bj] = std::isfinite(m) ? (float)(std::floor(m * 16.0) / 16.0) : 0.0f;
+      }
+    rc = x->alloc((void **)&x->datum, x->datum_host.size() * sizeof(float));
+    if (rc) return bad(rc);
+    if (hipMemcpy(x->datum, x->datum_host.data(), x->datum_host.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+      return bad(fail(SW2D_E_DEVICE, "datum upload"));
+  }
+  for (int i = 0; i < R; i++)
+    for (int j = 0; j < C; j++) {
+      size_t k = (size_t)i * C + j;
+      double zv = x->active[k] ? grid->z[k] : 0.0;
+      if (x->prec == SW2D_FP32 && x->active[k]) zv -= x->datum_host[(size_t)(i / 64) * x->datum_cols + j / 64];
+      zdev[k] = zv;
+    }
+  // write_pitched masks inactive cells only for cell state; bed is plain
+  {
+    std::vector<unsigned char> keep;
+    keep.swap(x->active);
+    x->active.assign((size_t)R * C, 1);
+    rc = x->write_pitched(x->z, R, C, zdev.data(), false);
+    x->active.swap(keep);
+    if (rc) return bad(rc);
+  }
+  if (grid->n_cell) {
+    rc = x->alloc(&x->ncell, cells * x->esz);
+    if (rc) return bad(rc);
+    std::vector<double> nv((size_t)R * C);
+    for (size_t k = 0; k < nv.size(); k++) {
+      const double n = grid->n_cell[k];
+      nv[k] = x->prec == SW2D_FP64 ? n : params->g * n * n;  // FP32 form stores g*n^2
+    }
+    rc = x->write_pitched(x->ncell, R, C, nv.data(), false);
+    if (rc) return bad(rc);
+  }
+  rc = x->write_pitched(x->h[0], R, C, grid->h0, true);
+  if (rc) return bad(rc);
+  for (int b = 0; b < 2; b++) {
+    if (hipMemset(x->U[b], 0, faces * x->esz) != hipSuccess || hipMemset(x->V[b], 0, cells * x->esz) != hipSuccess)
+      return bad(fail(SW2D_E_DEVICE, "state clear"));
+  }
+  if (hipMemcpy(x->h[1], x->h[0], cells * x->esz, hipMemcpyDeviceToDevice) != hipSuccess ||
+      hipMemset(x->Us, 0, faces * x->esz) != hipSuccess || hipMemset(x->Vs, 0, cells * x->esz) != hipSuccess ||
+      hipMemset(x->acc, 0, (size_t)x->n_tiles * 2 * sw2d::ACC_N * sizeof(double)) != hipSuccess)
+    return bad(fail(SW2D_E_DEVICE, "state init"));
+
+  Ctrl c;
+  std::memset(&c, 0, sizeof(c));
+  c.t = params->t0;
+  c.t_end = params->t_end > 0 ? params->t_end : std::numeric_limits<double>::infinity();
+  c.dt_setup = params->dt;
+  c.cfl = params->cfl;
+  c.forcing_mode = params->forcing_mode;
+  c.evaporation_on = params->evaporation_on;
+  c.dxdy = grid->dx * grid->dy;
+  c.dtrain = 1.0;
+  c.dtoq = 1.0;
+  rc = x->store_ctrl(c);
+  if (rc) return bad(rc);
+
+  auto fill = [&](auto &G) {
+    using T = std::remove_pointer_t<decltype(G.Us)>;
+    G.R = R;
+    G.C = C;
+    G.P = x->P;
+    G.tiles_x = x->tiles_x;
+    G.tiles_y = x->tiles_y;
+    G.edge_open = params->edge_open;
+    G.has_ncell = grid->n_cell != nullptr;
+    G.rain_block = 0;
+    G.rain_cols = 0;
+    G.datum_cols = x->datum_cols;
+    G.dx = grid->dx;
+    G.dy = grid->dy;
+    G.g = params->g;
+    G.th = params->th;
+    G.manning = params->manning;
+    G.INT = params->INT;
+    G.INF = params->INF;
+    G.LWL = params->LWL;
+    G.EVmin = params->EV_WL_min;
+    for (int b = 0; b < 2; b++) {
+      G.h[b] = (T *)x->h[b];
+      G.U[b] = (T *)x->U[b];
+      G.V[b] = (T *)x->V[b];
+    }
+    G.Us = (T *)x->Us;
+    G.Vs = (T *)x->Vs;
+    G.z = (const T *)x->z;
+    G.ncell = (const T *)x->ncell;
+    G.datum = x->datum;
+    G.rain_map = nullptr;
+    G.ctrl = x->d_ctrl;
+    G.acc = x->acc;
+    return hipMemcpy(x->d_grid, &G, sizeof(G), hipMemcpyHostToDevice);
+  };
+  hipError_t ge;
+  if (x->prec == SW2D_FP64) {
+    Grid<double> G{};
+    ge = fill(G);
+  } else {
+    Grid<float> G{};
+    ge = fill(G);
+  }
+  if (ge != hipSuccess) return bad(fail(SW2D_E_DEVICE, "grid descriptor upload"));
+  if (params->cfl > 0) {
+    rc = x->recompute_cfl();
+    if (rc) return bad(rc);
+  }
+  if (hipStreamSynchronize(x->stream) != hipSuccess) return bad(fail(SW2D_E_DEVICE, "init sync"));
+  *out = x;
+  return SW2D_OK;
+}
+
+void sw2d_destroy(sw2d_ctx *ctx) { delete ctx; }
+
+static int update_grid_rain(sw2d_ctx *x, double *map, int block, int cols) {
+  // patch the rain map fields of the device grid descriptor
+  if (x->prec == SW2D_FP64) {
+    Grid<double> G;
+    HIPCHK(hipMemcpy(&G, x->d_grid, sizeof(G), hipMemcpyDeviceToHost));
+    G.rain_map = map;
+    G.rain_block = block;
+    G.rain_cols = cols;
+    HIPCHK(hipMemcpy(x->d_grid, &G, sizeof(G), hipMemcpyHostToDevice));
+  } else {
+    Grid<float> G;
+    HIPCHK(hipMemcpy(&G, x->d_grid, sizeof(G), hipMemcpyDeviceToHost));
+    G.rain_map = map;
+    G.rain_block = block;
+    G.rain_cols = cols;
+    HIPCHK(hipMemcpy(x->d_grid, &G, sizeof(G), hipMemcpyHostToDevice));
+  }
+  return SW2D_OK;
+}
+
+int sw2d_set_rain(sw2d_ctx *x, const double *table, int n, double dtrain, const double *block_map, int block) {
+  if (!x || n < 0 || (n > 0 && !table)) return fail(SW2D_E_INVALID, "rain table");
+  if (n > 0 && !(dtrain > 0)) return fail(SW2D_E_INVALID, "dtrain must be > 0");
+  if (block_map && (block <= 0 || 64 % block != 0)) return fail(SW2D_E_INVALID, "rain block must divide 64");
+  Ctrl c;
+  int rc = x->fetch_ctrl(c);
+  if (rc) return rc;
+  x->release(x->d_rain);
+  x->d_rain = nullptr;
+  if (n > 0) {
+    rc = x->alloc((void **)&x->d_rain, (size_t)n * sizeof(double));
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(x->d_rain, table, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+  }
+  c.rain = x->d_rain;
+  c.n_rain = n;
+  if (n > 0) c.dtrain = dtrain;
+  rc = x->store_ctrl(c);
+  if (rc) return rc;
+  x->release(x->rain_map);
+  x->rain_map = nullptr;
+  int cols = 0;
+  if (block_map) {
+    const int br = (x->R + block - 1) / block;
+    cols = (x->C + block - 1) / block;
+    rc = x->alloc((void **)&x->rain_map, (size_t)br * cols * sizeof(double));
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(x->rain_map, block_map, (size_t)br * cols * sizeof(double), hipMemcpyHostToDevice));
+  }
+  return update_grid_rain(x, x->rain_map, block_map ? block : 0, cols);
+}
+
+int sw2d_set_sources(sw2d_ctx *x, const double *q, int n_rows, int nst, double dtoq, const int64_t *cell_index) {
+  if (!x || n_rows < 0 || nst < 0 || (n_rows * nst > 0 && !q) || (nst > 0 && !cell_index))
+    return fail(SW2D_E_INVALID, "source table");
+  if (nst > 0 && !(dtoq > 0)) return fail(SW2D_E_INVALID, "dtoq must be > 0");
+  // unique cells, the last source on a cell wins (assignment order L:545)
+  std::vector<long long> cells;
+  std::vector<int> cols;
+  const long long ncell = (long long)x->R * x->C;
+  for (int k = 0; k < nst; k++) {
+    const long long cidx = cell_index[k];
+    if (cidx < 0 || cidx >= ncell) continue;
+    auto it = std::find(cells.begin(), cells.end(), cidx);
+    if (it != cells.end()) cols[it - cells.begin()] = k;
+    else {
+      cells.push_back(cidx);
+      cols.push_back(k);
+    }
+  }
+  Ctrl c;
+  int rc = x->fetch_ctrl(c);
+  if (rc) return rc;
+  for (void *p : {(void *)x->d_q, (void *)x->d_src_col, (void *)x->d_src_cell, (void *)x->d_ql}) x->release(p);
+  x->d_q = x->d_ql = nullptr;
+  x->d_src_col = nullptr;
+  x->d_src_cell = nullptr;
+  const int ns = (int)cells.size();
+  if (ns > 0) {
+    rc = x->alloc((void **)&x->d_q, (size_t)n_rows * nst * sizeof(double));
+    rc = rc ? rc : x->alloc((void **)&x->d_src_col, ns * sizeof(int));
+    rc = rc ? rc : x->alloc((void **)&x->d_src_cell, ns * sizeof(long long));
+    rc = rc ? rc : x->alloc((void **)&x->d_ql, 2 * (size_t)ns * sizeof(double));
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(x->d_q, q, (size_t)n_rows * nst * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(x->d_src_col, cols.data(), ns * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(x->d_src_cell, cells.data(), ns * sizeof(long long), hipMemcpyHostToDevice));
+  }
+  c.q = x->d_q;
+  c.n_q = n_rows;
+  c.q_cols = nst;
+  c.dtoq = nst > 0 ? dtoq : 1.0;
+  c.nsrc = ns;
+  c.src_col = x->d_src_col;
+  c.src_cell = x->d_src_cell;
+  c.ql_a = x->d_ql;
+  c.ql_b = x->d_ql ? x->d_ql + ns : nullptr;
+  return x->store_ctrl(c);
+}
+
+int sw2d_set_evaporation(sw2d_ctx *x, const double *evapo, int n) {
+  if (!x || n < 0 || (n > 0 && !evapo)) return fail(SW2D_E_INVALID, "evaporation table");
+  Ctrl c;
+  int rc = x->fetch_ctrl(c);
+  if (rc) return rc;
+  x->release(x->d_evapo);
+  x->d_evapo = nullptr;
+  if (n > 0) {
+    rc = x->alloc((void **)&x->d_evapo, (size_t)n * sizeof(double));
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(x->d_evapo, evapo, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+  }
+  c.evapo = x->d_evapo;
+  c.n_evapo = n;
+  return x->store_ctrl(c);
+}
+
+int sw2d_set_dt_schedule(sw2d_ctx *x, const double *D, int n) {
+  if (!x || n < 0 || (n > 0 && !D)) return fail(SW2D_E_INVALID, "dt schedule");
+  for (int i = 0; i < n; i++)
+    if (!(D[i] > 0)) return fail(SW2D_E_INVALID, "dt schedule entries must be > 0");
+  Ctrl c;
+  int rc = x->fetch_ctrl(c);
+  if (rc) return rc;
+  x->release(x->d_dsched);
+  x->d_dsched = nullptr;
+  if (n > 0) {
+    rc = x->alloc((void **)&x->d_dsched, (size_t)n * sizeof(double));
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(x->d_dsched, D, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+  }
+  c.dsched = x->d_dsched;
+  c.n_dsched = n;
+  return x->store_ctrl(c);
+}
+
+int sw2d_record_dt(sw2d_ctx *x, int capacity) {
+  if (!x || capacity < 0) return fail(SW2D_E_INVALID, "dt history capacity");
+  Ctrl c;
+  int rc = x->fetch_ctrl(c);
+  if (rc) return rc;
+  x->release(x->d_dthist);
+  x->d_dthist = nullptr;
+  x->dthist_cap = 0;
+  if (capacity > 0) {
+    rc = x->alloc((void **)&x->d_dthist, (size_t)capacity * sizeof(double));
+    if (rc) return rc;
+    x->dthist_cap = capacity;
+  }
+  c.dt_hist = x->d_dthist;
+  c.dt_hist_cap = capacity;
+  return x->store_ctrl(c);
+}
+
+int sw2d_get_dt_history(sw2d_ctx *x, double *out, int capacity, int *n_out) {
+  if (!x || !out || !n_out) return fail(SW2D_E_INVALID, "dt history args");
+  Ctrl c;
+  int rc = x->fetch_ctrl(c);
+  if (rc) return rc;
+  const long long n = std::min<long long>(c.iter, x->dthist_cap);
+  if (capacity < n) return fail(SW2D_E_INVALID, "dt history buffer too small");
+  if (n > 0) HIPCHK(hipMemcpy(out, x->d_dthist, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+  *n_out = (int)n;
+  return SW2D_OK;
+}
+
+int sw2d_step(sw2d_ctx *x, int64_t n_iter) {
+  if (!x || n_iter < 0) return fail(SW2D_E_INVALID, "step count");
+  if (n_iter == 0) return SW2D_OK;
+  sw2d::IterLaunch a;
+  a.grids = x->d_grid;
+  a.work = nullptr;
+  a.n_work = x->n_tiles;
+  a.blocks = x->n_tiles;
+  a.ctrl = x->d_ctrl;
+  a.n_ctrl = 1;
+  a.ev = nullptr;
+  for (int64_t k = 0; k < n_iter; k++) {
+    if (x->prof_used < x->prof_cap) {
+      hipEvent_t *ev = &x->prof_ev[(size_t)4 * x->prof_used++];
+      HIPCHK(hipEventRecord(ev[0], x->stream));
+      a.ev = ev + 1;
+    } else {
+      a.ev = nullptr;
+    }
+    hipError_t e = x->prec == SW2D_FP64 ? sw2d::launch_iteration_f64(a, x->stream)
+                                        : sw2d::launch_iteration_f32(a, x->stream);
+    if (e != hipSuccess) return fail(SW2D_E_DEVICE, std::string("launch: ") + hipGetErrorString(e));
+  }
+  HIPCHK(sw2d::launch_control(x->d_ctrl, 1, 1, x->stream));
+  return SW2D_OK;
+}
+
+int sw2d_profile_begin(sw2d_ctx *x, int max_iters) {
+  if (!x || max_iters < 0) return fail(SW2D_E_INVALID, "profile capacity");
+  HIPCHK(hipStreamSynchronize(x->stream));
+  while ((int)x->prof_ev.size() < 4 * max_iters) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    x->prof_ev.push_back(e);
+  }
+  x->prof_cap = max_iters;
+  x->prof_used = 0;
+  return SW2D_OK;
+}
+
+int sw2d_profile_end(sw2d_ctx *x, double ms[3], int64_t *n_iter) {
+  if (!x || !ms || !n_iter) return fail(SW2D_E_INVALID, "null argument");
+  HIPCHK(hipStreamSynchronize(x->stream));
+  ms[0] = ms[1] = ms[2] = 0.0;
+  for (int k = 0; k < x->prof_used; k++) {
+    hipEvent_t *ev = &x->prof_ev[(size_t)4 * k];
+    for (int q = 0; q < 3; q++) {
+      float t = 0.0f;
+      HIPCHK(hipEventElapsedTime(&t, ev[q], ev[q + 1]));
+      ms[q] += t;
+    }
+  }
+  *n_iter = x->prof_used;
+  x->prof_cap = x->prof_used = 0;
+  return SW2D_OK;
+}
+
+int sw2d_sync(sw2d_ctx *x) {
+  if (!x) return fail(SW2D_E_INVALID, "null context");
+  Ctrl c;
+  int rc = x->fetch_ctrl(c);
+  if (rc) return rc;
+  if (c.error & sw2d::ERR_NONFINITE) return fail(SW2D_E_NONFINITE, "non-finite discharge in the state");
+  if (c.error & sw2d::ERR_TOO_MANY_SRC) return fail(SW2D_E_STATE, "too many point sources in one tile");
+  return SW2D_OK;
+}
+
+int sw2d_read(sw2d_ctx *x, int field, double *out) {
+  if (!x || !out) return fail(SW2D_E_INVALID, "null argument");
+  Ctrl c;
+  int rc = x->fetch_ctrl(c);
+  if (rc) return rc;
+  const int p = c.parity, R = x->R, C = x->C;
+  switch (field) {
+    case SW2D_FIELD_H:
+      rc = x->read_pitched(x->h[p], R, C, out);
+      if (!rc)
+        for (size_t k = 0; k < (size_t)R * C; k++)
+          if (out[k] < 0) out[k] = 0.0;
+      return rc;
+    case SW2D_FIELD_U:
+      return x->read_pitched(x->U[p], R + 1, C, out);
+    case SW2D_FIELD_V:
+      return x->read_pitched(x->V[p], R, C + 1, out);
+    case SW2D_FIELD_UU:
+    case SW2D_FIELD_VV: {
+      if (!x->d_uu) {
+        rc = x->alloc((void **)&x->d_uu, (size_t)R * C * sizeof(double));
+        rc = rc ? rc : x->alloc((void **)&x->d_vv, (size_t)R * C * sizeof(double));
+        if (rc) return rc;
+      }
+      HIPCHK(sw2d::launch_cell_velocity(x->prec, x->d_grid, 0, x->d_uu, x->d_vv, x->vol_blocks, x->stream));
+      HIPCHK(hipStreamSynchronize(x->stream));
+      HIPCHK(hipMemcpy(out, field == SW2D_FIELD_UU ? x->d_uu : x->d_vv, (size_t)R * C * sizeof(double),
+                       hipMemcpyDeviceToHost));
+      return SW2D_OK;
+    }
+    case SW2D_FIELD_Z:
+      rc = x->read_pitched(x->z, R, C, out);
+      if (!rc && x->prec == SW2D_FP32)
+        for (int i = 0; i < R; i++)
+          for (int j = 0; j < C; j++)
+            if (x->active[(size_t)i * C + j])
+              out[(size_t)i * C + j] += x->datum_host[(size_t)(i / 64) * x->datum_cols + j / 64];
+      if (!rc)
+        for (size_t k = 0; k < (size_t)R * C; k++)
+          if (!x->active[k]) out[k] = x->nodata;
+      return rc;
+    default:
+      return fail(SW2D_E_INVALID, "unknown field");
+  }
+}
+
+int sw2d_write_state(sw2d_ctx *x, const double *h, const double *U, const double *V) {
+  if (!x || !h || !U || !V) return fail(SW2D_E_INVALID, "null argument");
+  Ctrl c;
+  int rc = x->fetch_ctrl(c);
+  if (rc) return rc;
+  const int p = c.parity;
+  rc = x->write_pitched(x->h[p], x->R, x->C, h, true);
+  rc = rc ? rc : x->write_pitched(x->U[p], x->R + 1, x->C, U, false);
+  rc = rc ? rc : x->write_pitched(x->V[p], x->R, x->C + 1, V, false);
+  if (rc) return rc;
+  if (x->prm.cfl > 0) return x->recompute_cfl();
+  return SW2D_OK;
+}
+
+int sw2d_get_stats(sw2d_ctx *x, sw2d_stats *st) {
+  if (!x || !st) return fail(SW2D_E_INVALID, "null argument");
+  Ctrl c;
+  int rc = x->fetch_ctrl(c);
+  if (rc) return rc;
+  HIPCHK(sw2d::launch_volume(x->prec, x->d_grid, 0, x->partial, x->vol_blocks, x->stream));
+  std::vector<double> part((size_t)x->vol_blocks * 2);
+  HIPCHK(hipStreamSynchronize(x->stream));
+  HIPCHK(hipMemcpy(part.data(), x->partial, part.size() * sizeof(double), hipMemcpyDeviceToHost));
+  Comp vol;
+  for (int b = 0; b < x->vol_blocks; b++) {
+    vol.add(part[2 * b]);
+    vol.add(part[2 * b + 1]);
+  }
+  std::vector<double> acc((size_t)x->n_tiles * 2 * sw2d::ACC_N);
+  HIPCHK(hipMemcpy(acc.data(), x->acc, acc.size() * sizeof(double), hipMemcpyDeviceToHost));
+  Comp q[sw2d::ACC_N];
+  for (int t = 0; t < x->n_tiles; t++)
+    for (int k = 0; k < sw2d::ACC_N; k++) {
+      q[k].add(acc[(size_t)t * 2 * sw2d::ACC_N + 2 * k]);
+      q[k].add(acc[(size_t)t * 2 * sw2d::ACC_N + 2 * k + 1]);
+    }
+  const double a = x->dx * x->dy;  // depth-equivalent sums -> m3
+  st->t = c.t;
+  st->iterations = c.iter;
+  st->last_D = c.D;
+  st->done = c.done;
+  st->volume = vol.value() * a;
+  st->rain_in = q[0].value() * a;
+  st->source_in = q[1].value() * a;
+  st->evap_out = q[2].value() * a;
+  st->outflow = q[3].value() * a;
+  st->clamp_gain = q[4].value() * a;
+  return SW2D_OK;
+}
+
+namespace {
+struct SnapHeader {
+  uint32_t magic, version;
+  int32_t prec, R, C, done;
+  double t;
+  int64_t iter;
+  uint64_t smax_bits;
+  double last_D;
+};
+constexpr uint32_t SNAP_MAGIC = 0x44325753u;  // "SW2D"
+}  // namespace
+
+int sw2d_snapshot(sw2d_ctx *x, void *buf, size_t *size) {
+  if (!x || !size) return fail(SW2D_E_INVALID, "null argument");
+  const size_t nh = (size_t)x->R * x->C, nu = (size_t)(x->R + 1) * x->C, nv = (size_t)x->R * (x->C + 1);
+  const size_t nacc = (size_t)x->n_tiles * 2 * sw2d::ACC_N;
+  const size_t need = sizeof(SnapHeader) + (nh + nu + nv) * x->esz + nacc * sizeof(double);
+  if (!buf) {
+    *size = need;
+    return SW2D_OK;
+  }
+  if (*size < need) return fail(SW2D_E_INVALID, "snapshot buffer too small");
+  Ctrl c;
+  int rc = x->fetch_ctrl(c);
+  if (rc) return rc;
+  SnapHeader hd{SNAP_MAGIC, 1, x->prec, x->R, x->C, c.done, c.t, c.iter, c.smax_bits, c.D};
+  unsigned char *o = (unsigned char *)buf;
+  std::memcpy(o, &hd, sizeof(hd));
+  o += sizeof(hd);
+  auto grab = [&](void *src, int rows, int cols) -> int {
+    std::vector<unsigned char> tmp((size_t)rows * x->P * x->esz);
+    HIPCHK(hipMemcpy(tmp.data(), src, tmp.size(), hipMemcpyDeviceToHost));
+    for (int i = 0; i < rows; i++) {
+      std::memcpy(o, tmp.data() + (size_t)i * x->P * x->esz, (size_t)cols * x->esz);
+      o += (size_t)cols * x->esz;
+    }
+    return SW2D_OK;
+  };
+  rc = grab(x->h[c.parity], x->R, x->C);
+  rc = rc ? rc : grab(x->U[c.parity], x->R + 1, x->C);
+  rc = rc ? rc : grab(x->V[c.parity], x->R, x->C + 1);
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(o, x->acc, nacc * sizeof(double), hipMemcpyDeviceToHost));
+  *size = need;
+  return SW2D_OK;
+}
+
+int sw2d_restore(sw2d_ctx *x, const void *buf, size_t size) {
+  if (!x || !buf) return fail(SW2D_E_INVALID, "null argument");
+  SnapHeader hd;
+  if (size < sizeof(hd)) return fail(SW2D_E_INVALID, "snapshot too small");
+  std::memcpy(&hd, buf, sizeof(hd));
+  if (hd.magic != SNAP_MAGIC || hd.prec != x->prec || hd.R != x->R || hd.C != x->C)
+    return fail(SW2D_E_INVALID, "snapshot does not match this context");
+  const size_t nh = (size_t)x->R * x->C, nu = (size_t)(x->R + 1) * x->C, nv = (size_t)x->R * (x->C + 1);
+  const size_t nacc = (size_t)x->n_tiles * 2 * sw2d::ACC_N;
+  if (size < sizeof(hd) + (nh + nu + nv) * x->esz + nacc * sizeof(double))
+    return fail(SW2D_E_INVALID, "snapshot truncated");
+  Ctrl c;
+  int rc = x->fetch_ctrl(c);
+  if (rc) return rc;
+  const unsigned char *in = (const unsigned char *)buf + sizeof(hd);
+  auto put = [&](void *dst, int rows, int cols) -> int {
+    std::vector<unsigned char> tmp((size_t)rows * x->P * x->esz, 0);
+    for (int i = 0; i < rows; i++) {
+      std::memcpy(tmp.data() + (size_t)i * x->P * x->esz, in, (size_t)cols * x->esz);
+      in += (size_t)cols * x->esz;
+    }
+    HIPCHK(hipMemcpy(dst, tmp.data(), tmp.size(), hipMemcpyHostToDevice));
+    return SW2D_OK;
+  };
+  rc = put(x->h[0], x->R, x->C);
+  rc = rc ? rc : put(x->U[0], x->R + 1, x->C);
+  rc = rc ? rc : put(x->V[0], x->R, x->C + 1);
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(x->acc, in, nacc * sizeof(double), hipMemcpyHostToDevice));
+  c.parity = 0;
+  c.run = 0;
+  c.t = hd.t;
+  c.iter = hd.iter;
+  c.done = hd.done;
+  c.smax_bits = hd.smax_bits;
+  c.D = hd.last_D;
+  c.error = 0;
+  return x->store_ctrl(c);
+}
+
+}  // extern "C"

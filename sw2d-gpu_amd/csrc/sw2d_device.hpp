@@ -1,0 +1,110 @@
+// sw2d_device.hpp -- device-side data structures shared by the step kernels
+// and the host runtime of the MI355X SW2D core.
+//
+// Memory layout in HBM (one scenario):
+//   all arrays row-major with one pitch P >= cols+1 (multiple of 64 elements)
+//   h[2]      rows   x P   depth, ping-pong; inactive cells hold -1
+//   U[2]      rows+1 x P   discharge on U faces (row i = face north of cell i)
+//   V[2]      rows   x P   discharge on V faces (column j = face west of j,
+//                          column cols = east boundary face)
+//   Us, Vs    momentum-stage output of the current iteration
+//   z         rows   x P   bed (FP32 form: minus a per-64x64-block datum)
+//   ncell     rows   x P   optional Manning raster (FP32 form: g*n^2)
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sw2d {
+
+constexpr int TILE_W = 64;          // tile width: one wavefront per tile row
+constexpr int BLOCK = 256;          // threads per workgroup (4 wavefronts)
+constexpr int MAX_TILE_SRC = 64;    // point sources inside one tile region
+constexpr int DATUM_BLOCK = 64;     // FP32 bed datum block (cells)
+constexpr int ACC_N = 5;            // rain, source, evap, outflow, clamp
+constexpr int EDGE_N = 1, EDGE_S = 2, EDGE_W = 4, EDGE_E = 8;
+constexpr int ERR_NONFINITE = 1, ERR_TOO_MANY_SRC = 2;
+
+// Per-scenario control block, resident in device memory.  The control
+// kernel fills the per-iteration fields; the stencil kernels only read them.
+struct Ctrl {
+  // clock
+  double t;            // time at the start of the pending iteration
+  double t_next;       // time after the pending iteration
+  double t_end;        // loop runs while t + dth <= t_end (L:1256)
+  double dt_setup;     // setup.dat dt
+  double cfl;          // 0: fixed step
+  // values of the pending iteration
+  double D, dth;
+  double rr_a, rr_b;   // rain rate [m/s] at t and t + dth
+  double ev_a, ev_b;   // evaporation rate [m/s]
+  long long iter;      // committed iterations
+  int parity;          // h/U/V buffer holding the committed state
+  int run;             // 1 while an iteration is pending
+  int done;            // end condition reached
+  int error;           // ERR_* bits
+  unsigned long long smax_bits;  // CFL speed max of the newest state
+  // forcing tables (device pointers)
+  const double *rain; int n_rain; double dtrain;
+  const double *evapo; int n_evapo; int evaporation_on;
+  const double *q; int n_q; int q_cols; double dtoq;
+  int nsrc;                      // unique source cells
+  const int *src_col;            // Q-table column of each source
+  const long long *src_cell;     // linear cell index i*cols + j
+  double *ql_a, *ql_b;           // source rates [m/s] at t and t + dth
+  const double *dsched; int n_dsched;
+  double *dt_hist; int dt_hist_cap;
+  int forcing_mode;
+  double dxdy;
+};
+
+template <typename T>
+struct Grid {
+  int R, C, P;
+  int tiles_x, tiles_y;
+  int edge_open;
+  int has_ncell;
+  int rain_block, rain_cols;     // rain multiplier map geometry (0 = none)
+  int datum_cols;                // FP32: datum blocks per row
+  double dx, dy, g, th, manning;
+  double INT, INF, LWL, EVmin;
+  T *h[2];
+  T *U[2];
+  T *V[2];
+  T *Us, *Vs;
+  const T *z;
+  const T *ncell;
+  const float *datum;            // FP32: datum per 64x64 block
+  const double *rain_map;
+  Ctrl *ctrl;
+  double *acc;                   // per tile: ACC_N x (sum, compensation)
+};
+
+// Interpolation shared by rain, point sources and evaporation
+// (stream_flow L:533-538, L:547-552; evaporation_load L:592-597).
+// mode 1: piecewise constant.  Indices past the table hold the last entry.
+__host__ __device__ inline double forcing_value(double time, const double *table,
+                                                int n, double dT, int mode) {
+  if (n <= 0) return 0.0;
+  if (mode == 1) {
+    int it = (int)(time / dT);
+    it = it < 0 ? 0 : (it > n - 1 ? n - 1 : it);
+    return table[it];
+  }
+  if (time <= 1.00) return table[0] * time;
+  int it = (int)(time / dT);
+  int i0 = it < 0 ? 0 : (it > n - 1 ? n - 1 : it);
+  int i1 = it + 1 < 0 ? 0 : (it + 1 > n - 1 ? n - 1 : it + 1);
+  return table[i0] + (table[i1] - table[i0]) / (dT * (time - dT * (it)));
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous run of tiles so stencil halos shared by neighbouring tiles
+// stay in that XCD's L2.
+__device__ inline int xcd_tile(int b, int n) {
+  int q = n >> 3;
+  if (b >= (q << 3)) return b;
+  return (b & 7) * q + (b >> 3);
+}
+
+}  // namespace sw2d

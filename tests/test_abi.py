@@ -1,0 +1,103 @@
+"""The C ABI library loads on a machine without a GPU, exports every entry
+point include/sw2d.h declares, and its host-only parts (case reader, VTK
+writer, forcing formulas) agree with the oracle's restatements."""
+import gzip
+import os
+import re
+import shutil
+
+import numpy as np
+import pytest
+
+from conftest import DAM_BREAK, ROOT, ensure_oracle
+
+ensure_oracle()
+import caseio  # noqa: E402
+import oracle_py as O  # noqa: E402
+import sw2d  # noqa: E402
+
+
+def declared_symbols():
+    names = set()
+    for h in os.listdir(os.path.join(ROOT, "include")):
+        if h.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", h)).read()
+            names |= set(re.findall(r"\b(sw2d_[a-z_0-9]+)\s*\(", txt))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    L = sw2d.lib()
+    names = declared_symbols()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(names) == set(sw2d.EXPORTS)
+
+
+def test_case_reader_matches_restatement(dam_case):
+    c = sw2d.Case(DAM_BREAK)
+    I = c.info
+    ref = dam_case
+    assert (I.rows, I.cols) == ref["dem"]["z"].shape
+    assert I.cellsize == ref["dem"]["res"] and I.nodata == ref["dem"]["nodata"]
+    assert I.dt == ref["setup"]["dt"] and I.manning_coef == ref["setup"]["manning_coef"]
+    assert I.timmax == ref["timmax"] == 6000.0
+    assert np.array_equal(c.array(sw2d.CASE_DEM), ref["dem"]["z"].ravel())
+    assert np.array_equal(c.array(sw2d.CASE_IC), ref["ic"].ravel())
+    assert np.array_equal(c.array(sw2d.CASE_RAIN), ref["rain"])
+    assert np.array_equal(c.array(sw2d.CASE_Q), ref["q"].ravel())
+    assert I.n_q_rows == 817 and I.n_rain == 120 and I.dtrain == 60.0
+    assert list(c.source_cells()) == caseio.source_cells(ref) == [10001]
+
+
+def test_vtk_writer_bytes_match_reference_output(dam_case, tmp_path):
+    """The product writer, fed the oracle's first-iteration velocities,
+    reproduces the reference's committed Results_0.vtk byte for byte."""
+    c = dam_case
+    d, s = c["dem"], c["setup"]
+    act = (d["z"] != d["nodata"]).astype(np.uint8)
+    o = O.OracleSim(d["z"], act, c["ic"], d["res"], s["dt"], s["manning_coef"])
+    o.step(1)
+    case = sw2d.Case(DAM_BREAK)
+    out = str(tmp_path / "Results_0.vtk")
+    case.write_vtk(out, c["ic"], o.read(O.VV), o.read(O.UU), d["z"])
+    ref = gzip.open(os.path.join(DAM_BREAK, "output", "Results_0.vtk.gz")).read()
+    assert open(out, "rb").read() == ref
+
+
+def test_forcing_formulas_match_oracle():
+    a = np.array([2.0, 4.0, 8.0, 8.0, 1.0])
+    for t in [0.0, 0.3, 1.0, 1.5, 59.9, 60.0, 61.0, 125.5, 299.0, 1e6]:
+        for mode in (0, 1):
+            x, y = sw2d.forcing_value(t, a, 60.0, mode), O.stream_rate(t, a, 60.0, mode)
+            assert (x == y) or (np.isnan(x) and np.isnan(y))
+    T = np.linspace(5, 35, 31)
+    Rg = np.linspace(-50, 900, 31)
+    e = sw2d.evaporation_table(0.25, T, Rg, 3600.0)
+    assert np.array_equal(e, [O.evap_entry(0.25, t, r, 3600.0) for t, r in zip(T, Rg)])
+
+
+def test_missing_file_is_an_io_error(tmp_path):
+    d = tmp_path / "case"
+    shutil.copytree(DAM_BREAK, d)
+    os.remove(d / "input" / "rain.dat")
+    with pytest.raises(sw2d.SW2DError) as e:
+        sw2d.Case(str(d))
+    assert e.value.code == sw2d.E_IO and "rain.dat" in str(e.value)
+
+
+def test_malformed_setup_is_rejected(tmp_path):
+    d = tmp_path / "case"
+    shutil.copytree(DAM_BREAK, d)
+    p = d / "input" / "setup.dat"
+    p.write_bytes(p.read_bytes().replace(b"dpout:", b"dpoutx:"))
+    with pytest.raises(sw2d.SW2DError) as e:
+        sw2d.Case(str(d))
+    assert e.value.code == sw2d.E_IO
+
+
+def test_footprint_is_about_40_bytes_per_cell_fp32():
+    b = sw2d.footprint_bytes(8192, 8192, sw2d.FP32, True)
+    per = b / (8192 * 8192)
+    assert 40 <= per <= 46
