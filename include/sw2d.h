@@ -169,6 +169,44 @@ int sw2d_evaporation_table(double albedo, const double *T, const double *Rg,
 double sw2d_forcing_value(double time, const double *table, int n, double dT,
                           int mode);
 
+/* ---- batched ensembles (no reference counterpart: the reference runs the
+ * cases of dir.txt one after another, L:775-1526) ---------------------- */
+/* Many scenarios, each a full sw2d_ctx, advanced together: one launch per
+ * stage covers every live scenario (a work-list of 64-column strip segments
+ * of `rows` rows, chosen per scenario), and chunks of iterations replay a
+ * captured HIP graph.  Every scenario keeps its own clock, step D and end
+ * time. */
+typedef struct sw2d_batch sw2d_batch;
+int sw2d_batch_create(int precision, sw2d_batch **out);
+void sw2d_batch_destroy(sw2d_batch *b);
+/* Adds a scenario (same arguments as sw2d_create, params->precision must
+ * match).  share_with >= 0 names an earlier scenario with the same bed and
+ * Manning raster: its static arrays are shared, and tiles of the two are
+ * scheduled next to each other so the statics are read once from HBM. */
+int sw2d_batch_add(sw2d_batch *b, const sw2d_grid *grid, const sw2d_params *params,
+                   int share_with, int *id_out);
+/* The scenario's context (for set_rain/read/stats/snapshot).  Waits for
+ * the batch stream.  Changing forcing tables needs sw2d_batch_touch. */
+int sw2d_batch_scenario(sw2d_batch *b, int id, sw2d_ctx **ctx);
+int sw2d_batch_touch(sw2d_batch *b);
+/* Removes a scenario from the work-list and frees its device memory. */
+int sw2d_batch_remove(sw2d_batch *b, int id);
+/* Segment length (rows) of one scenario's work items: a multiple of 4 in
+ * [4, 4096]; 0 = the autotuned choice for its size bucket. */
+int sw2d_batch_set_tile_rows(sw2d_batch *b, int id, int rows);
+/* Times every candidate segment length (8, 32, 128, 512 rows) on the
+ * largest scenario of each size bucket (floor(log2(cells)/2)) for `iters`
+ * iterations, keeps the fastest per bucket, and restores the probed
+ * scenarios bitwise. */
+int sw2d_batch_autotune(sw2d_batch *b, int iters);
+int sw2d_batch_tile_rows(sw2d_batch *b, int id, int *rows);
+int sw2d_batch_step(sw2d_batch *b, int64_t n_iter);
+int sw2d_batch_sync(sw2d_batch *b);
+/* done[k] = 1 when scenario k (ids 0..n-1, removed ones report 1) has
+ * reached its t_end; iters[k] = its committed iterations (either may be
+ * NULL).  Synchronises. */
+int sw2d_batch_status(sw2d_batch *b, int *done, int64_t *iters, int cap, int *n_out);
+
 /* ---- reference case directories (db/<case>/input, output) ------------ */
 typedef struct sw2d_case sw2d_case;
 

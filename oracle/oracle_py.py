@@ -52,6 +52,7 @@ def lib():
         L.orc_set_evaporation.argtypes = [C.c_void_p, dp, C.c_int]
         L.orc_set_dt_schedule.argtypes = [C.c_void_p, dp, C.c_int]
         L.orc_step.argtypes = [C.c_void_p, C.c_int]
+        L.orc_set_state.argtypes = [C.c_void_p, dp, dp, dp]
         L.orc_read.argtypes = [C.c_void_p, C.c_int, dp]
         L.orc_get_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
         L.orc_stream_rate.restype = C.c_double
@@ -116,6 +117,10 @@ class OracleSim:
     def set_dt_schedule(self, D):
         d = np.ascontiguousarray(D, dtype=np.float64)
         lib().orc_set_dt_schedule(self.h_, _dp(d), len(d))
+
+    def set_state(self, h, U, V):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (h, U, V)]
+        lib().orc_set_state(self.h_, *[_dp(a) for a in arrs])
 
     def step(self, n):
         if lib().orc_step(self.h_, int(n)) != 0:

@@ -682,6 +682,20 @@ static void iteration(orc_sim *s, double *ql) {
   s->iters++;
 }
 
+int orc_set_state(orc_sim *s, const double *h, const double *U, const double *V) {
+  size_t nc = (size_t)s->R * s->C, nu = (size_t)(s->R + 1) * s->C,
+         nv = (size_t)s->R * (s->C + 1);
+  for (size_t c = 0; c < nc; c++) s->h[c] = s->act[c] ? h[c] : 0.0;
+  memcpy(s->U, U, nu * sizeof(double));
+  memcpy(s->V, V, nv * sizeof(double));
+  derive_faces(s);
+  cell_velocity(s);
+  memcpy(s->Uo, s->U, nu * sizeof(double));
+  memcpy(s->Vo, s->V, nv * sizeof(double));
+  memcpy(s->ho, s->h, nc * sizeof(double));
+  return 0;
+}
+
 int orc_step(orc_sim *s, int n_iter) {
   if (!s || n_iter < 0) return -1;
   double *ql = dalloc((size_t)s->R * s->C); /* h_ql, zero-initialised L:1149 */

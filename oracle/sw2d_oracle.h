@@ -64,6 +64,10 @@ int orc_set_evaporation(orc_sim *s, const double *evapo, int n);
 /* replay a fixed schedule of D values (overrides CFL) */
 int orc_set_dt_schedule(orc_sim *s, const double *D, int n);
 
+/* overwrite the state (h: R*C, U: (R+1)*C, V: R*(C+1)); the derived face
+ * fields are rebuilt from it, inactive cells keep h = 0 */
+int orc_set_state(orc_sim *s, const double *h, const double *U, const double *V);
+
 /* run n iterations; each iteration = one update of step D */
 int orc_step(orc_sim *s, int n_iter);
 

@@ -6,156 +6,13 @@
 // 9 launches + 9 device syncs + 4 H2D copies per iteration become ping-pong
 // state buffers, device-resident forcing tables and three stream-ordered
 // launches per iteration with no host synchronisation.
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <cmath>
-#include <cstdlib>
-#include <cstring>
-#include <limits>
-#include <string>
-#include <vector>
-
-#include "sw2d.h"
-#include "sw2d_launch.hpp"
-
-using sw2d::Ctrl;
-using sw2d::Grid;
+#include "sw2d_internal.hpp"
 
 namespace {
-
 thread_local std::string g_err;
-
-int fail(int code, const std::string &msg) {
-  g_err = msg;
-  return code;
-}
-
-#define HIPCHK(x)                                                                        \
-  do {                                                                                   \
-    hipError_t e_ = (x);                                                                 \
-    if (e_ != hipSuccess) return fail(SW2D_E_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; }
-
-struct Comp {
-  double s = 0, c = 0;
-  void add(double x) {
-    double t = s + x;
-    if (std::fabs(s) >= std::fabs(x)) c += (s - t) + x;
-    else c += (x - t) + s;
-    s = t;
-  }
-  double value() const { return s + c; }
-};
-
 }  // namespace
 
 void sw2d_set_error_text(const std::string &m) { g_err = m; }
-
-struct sw2d_ctx {
-  int prec = 0;
-  size_t esz = 8;
-  int R = 0, C = 0, P = 0, TH = 0, tiles_x = 0, tiles_y = 0, n_tiles = 0;
-  sw2d_params prm{};
-  double nodata = 0;
-  double dx = 0, dy = 0;
-  hipStream_t stream = nullptr;
-  std::vector<void *> allocs;
-  void *h[2]{}, *U[2]{}, *V[2]{}, *Us = nullptr, *Vs = nullptr, *z = nullptr, *ncell = nullptr;
-  float *datum = nullptr;
-  double *rain_map = nullptr, *acc = nullptr, *partial = nullptr;
-  double *d_uu = nullptr, *d_vv = nullptr;
-  Ctrl *d_ctrl = nullptr;
-  void *d_grid = nullptr;
-  double *d_rain = nullptr, *d_evapo = nullptr, *d_q = nullptr, *d_dsched = nullptr,
-         *d_dthist = nullptr, *d_ql = nullptr;
-  int *d_src_col = nullptr;
-  long long *d_src_cell = nullptr;
-  int vol_blocks = 0;
-  int dthist_cap = 0;
-  std::vector<unsigned char> active;  // rows*cols
-  std::vector<float> datum_host;
-  int datum_cols = 0;
-  // kernel timing: 4 events per iteration (before control, after control,
-  // after momentum, after continuity)
-  std::vector<hipEvent_t> prof_ev;
-  int prof_cap = 0, prof_used = 0;
-
-  ~sw2d_ctx() {
-    if (stream) (void)hipStreamSynchronize(stream);
-    for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
-    for (void *p : allocs) (void)hipFree(p);
-    if (stream) (void)hipStreamDestroy(stream);
-  }
-
-  int alloc(void **p, size_t bytes) {
-    hipError_t e = hipMalloc(p, bytes ? bytes : 8);
-    if (e != hipSuccess) {
-      *p = nullptr;
-      return fail(SW2D_E_MEMORY, std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
-    }
-    allocs.push_back(*p);
-    return SW2D_OK;
-  }
-  void release(void *p) {
-    if (!p) return;
-    auto it = std::find(allocs.begin(), allocs.end(), p);
-    if (it != allocs.end()) allocs.erase(it);
-    (void)hipFree(p);
-  }
-
-  int fetch_ctrl(Ctrl &c) {
-    HIPCHK(hipStreamSynchronize(stream));
-    HIPCHK(hipMemcpy(&c, d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
-    return SW2D_OK;
-  }
-  int store_ctrl(const Ctrl &c) {
-    HIPCHK(hipMemcpy(d_ctrl, &c, sizeof(Ctrl), hipMemcpyHostToDevice));
-    return SW2D_OK;
-  }
-  int blocks_for(long long n) const {
-    long long b = (n + sw2d::BLOCK - 1) / sw2d::BLOCK;
-    return (int)std::max(1LL, std::min(b, 4096LL));
-  }
-  int recompute_cfl() {
-    Ctrl c;
-    int rc = fetch_ctrl(c);
-    if (rc) return rc;
-    c.smax_bits = 0;
-    rc = store_ctrl(c);
-    if (rc) return rc;
-    HIPCHK(sw2d::launch_cfl_state(prec, d_grid, 0, blocks_for((long long)R * C), stream));
-    HIPCHK(hipStreamSynchronize(stream));
-    return SW2D_OK;
-  }
-  // copy a device array with pitch P into a dense host double array
-  int read_pitched(const void *src, int rows, int cols, double *out) {
-    std::vector<unsigned char> buf((size_t)rows * P * esz);
-    HIPCHK(hipMemcpy(buf.data(), src, buf.size(), hipMemcpyDeviceToHost));
-    for (int i = 0; i < rows; i++)
-      for (int j = 0; j < cols; j++) {
-        size_t k = (size_t)i * P + j;
-        out[(size_t)i * cols + j] = prec == SW2D_FP64 ? ((const double *)buf.data())[k]
-                                                      : (double)((const float *)buf.data())[k];
-      }
-    return SW2D_OK;
-  }
-  int write_pitched(void *dst, int rows, int cols, const double *in, bool cells) {
-    std::vector<unsigned char> buf((size_t)rows * P * esz, 0);
-    for (int i = 0; i < rows; i++)
-      for (int j = 0; j < cols; j++) {
-        double v = in[(size_t)i * cols + j];
-        if (cells && !active[(size_t)i * C + j]) v = -1.0;
-        size_t k = (size_t)i * P + j;
-        if (prec == SW2D_FP64) ((double *)buf.data())[k] = v;
-        else ((float *)buf.data())[k] = (float)v;
-      }
-    HIPCHK(hipMemcpy(dst, buf.data(), buf.size(), hipMemcpyHostToDevice));
-    return SW2D_OK;
-  }
-};
 
 extern "C" {
 
@@ -174,8 +31,7 @@ void sw2d_params_default(sw2d_params *p) {
 size_t sw2d_footprint_bytes(int rows, int cols, int precision, int has_n) {
   const size_t esz = precision == SW2D_FP64 ? 8 : 4;
   const size_t P = round_up((size_t)cols + 1, 64);
-  const int TH = precision == SW2D_FP64 ? sw2d::TILE_H_F64 : sw2d::TILE_H_F32;
-  const size_t tiles = ((size_t)(cols + 63) / 64) * (((size_t)rows + TH - 1) / TH);
+  const size_t tiles = ((size_t)(cols + 63) / 64) * (((size_t)rows + 3) / 4);
   size_t cells = (size_t)rows * P, faces = (size_t)(rows + 1) * P;
   size_t b = esz * (2 * cells + 2 * faces + 2 * cells + faces + cells + cells);
   if (has_n) b += esz * cells;
@@ -228,10 +84,11 @@ int sw2d_create(const sw2d_grid *grid, const sw2d_params *params, sw2d_ctx **out
   x->R = R;
   x->C = C;
   x->P = (int)round_up((size_t)C + 1, 64);
-  x->TH = x->prec == SW2D_FP64 ? sw2d::TILE_H_F64 : sw2d::TILE_H_F32;
   x->tiles_x = (C + sw2d::TILE_W - 1) / sw2d::TILE_W;
+  x->TH = sw2d::default_seg_rows(R, x->tiles_x);
   x->tiles_y = (R + x->TH - 1) / x->TH;
   x->n_tiles = x->tiles_x * x->tiles_y;
+  x->n_acc = x->tiles_x * ((R + 3) / 4);
   x->prm = *params;
   x->nodata = grid->nodata;
   x->dx = grid->dx;
@@ -253,10 +110,11 @@ int sw2d_create(const sw2d_grid *grid, const sw2d_params *params, sw2d_ctx **out
   rc = rc ? rc : x->alloc(&x->Us, faces * x->esz);
   rc = rc ? rc : x->alloc(&x->Vs, cells * x->esz);
   rc = rc ? rc : x->alloc(&x->z, cells * x->esz);
-  rc = rc ? rc : x->alloc((void **)&x->acc, (size_t)x->n_tiles * 2 * sw2d::ACC_N * sizeof(double));
+  rc = rc ? rc : x->alloc((void **)&x->acc, (size_t)x->n_acc * 2 * sw2d::ACC_N * sizeof(double));
   x->vol_blocks = x->blocks_for((long long)R * C);
   rc = rc ? rc : x->alloc((void **)&x->partial, (size_t)x->vol_blocks * 2 * sizeof(double));
   rc = rc ? rc : x->alloc((void **)&x->d_ctrl, sizeof(Ctrl));
+  rc = rc ? rc : x->alloc((void **)&x->d_ctrl_list, sizeof(Ctrl *));
   rc = rc ? rc : x->alloc(&x->d_grid, x->prec == SW2D_FP64 ? sizeof(Grid<double>) : sizeof(Grid<float>));
   if (rc) return bad(rc);
 
@@ -318,9 +176,11 @@ int sw2d_create(const sw2d_grid *grid, const sw2d_params *params, sw2d_ctx **out
   }
   if (hipMemcpy(x->h[1], x->h[0], cells * x->esz, hipMemcpyDeviceToDevice) != hipSuccess ||
       hipMemset(x->Us, 0, faces * x->esz) != hipSuccess || hipMemset(x->Vs, 0, cells * x->esz) != hipSuccess ||
-      hipMemset(x->acc, 0, (size_t)x->n_tiles * 2 * sw2d::ACC_N * sizeof(double)) != hipSuccess)
+      hipMemset(x->acc, 0, (size_t)x->n_acc * 2 * sw2d::ACC_N * sizeof(double)) != hipSuccess)
     return bad(fail(SW2D_E_DEVICE, "state init"));
 
+  if (hipMemcpy(x->d_ctrl_list, &x->d_ctrl, sizeof(Ctrl *), hipMemcpyHostToDevice) != hipSuccess)
+    return bad(fail(SW2D_E_DEVICE, "control list upload"));
   Ctrl c;
   std::memset(&c, 0, sizeof(c));
   c.t = params->t0;
@@ -342,6 +202,7 @@ int sw2d_create(const sw2d_grid *grid, const sw2d_params *params, sw2d_ctx **out
     G.P = x->P;
     G.tiles_x = x->tiles_x;
     G.tiles_y = x->tiles_y;
+    G.seg_rows = x->TH;
     G.edge_open = params->edge_open;
     G.has_ncell = grid->n_cell != nullptr;
     G.rain_block = 0;
@@ -392,6 +253,8 @@ int sw2d_create(const sw2d_grid *grid, const sw2d_params *params, sw2d_ctx **out
 void sw2d_destroy(sw2d_ctx *ctx) { delete ctx; }
 
 static int update_grid_rain(sw2d_ctx *x, double *map, int block, int cols) {
+  int shift = 0;
+  while (block > 0 && (1 << shift) < block) shift++;
   // patch the rain map fields of the device grid descriptor
   if (x->prec == SW2D_FP64) {
     Grid<double> G;
@@ -399,6 +262,7 @@ static int update_grid_rain(sw2d_ctx *x, double *map, int block, int cols) {
     G.rain_map = map;
     G.rain_block = block;
     G.rain_cols = cols;
+    G.rain_shift = shift;
     HIPCHK(hipMemcpy(x->d_grid, &G, sizeof(G), hipMemcpyHostToDevice));
   } else {
     Grid<float> G;
@@ -406,6 +270,7 @@ static int update_grid_rain(sw2d_ctx *x, double *map, int block, int cols) {
     G.rain_map = map;
     G.rain_block = block;
     G.rain_cols = cols;
+    G.rain_shift = shift;
     HIPCHK(hipMemcpy(x->d_grid, &G, sizeof(G), hipMemcpyHostToDevice));
   }
   return SW2D_OK;
@@ -565,22 +430,26 @@ int sw2d_step(sw2d_ctx *x, int64_t n_iter) {
   a.work = nullptr;
   a.n_work = x->n_tiles;
   a.blocks = x->n_tiles;
-  a.ctrl = x->d_ctrl;
+  a.ctrl = x->d_ctrl_list;
   a.n_ctrl = 1;
   a.ev = nullptr;
-  for (int64_t k = 0; k < n_iter; k++) {
-    if (x->prof_used < x->prof_cap) {
-      hipEvent_t *ev = &x->prof_ev[(size_t)4 * x->prof_used++];
-      HIPCHK(hipEventRecord(ev[0], x->stream));
-      a.ev = ev + 1;
-    } else {
-      a.ev = nullptr;
+  if (x->prof_used >= x->prof_cap) {
+    int rc = enqueue_iterations(x->prec, x->stream, a, n_iter, &x->graph);
+    if (rc) return rc;
+  } else {
+    for (int64_t k = 0; k < n_iter; k++) {
+      if (x->prof_used < x->prof_cap) {
+        hipEvent_t *ev = &x->prof_ev[(size_t)4 * x->prof_used++];
+        HIPCHK(hipEventRecord(ev[0], x->stream));
+        a.ev = ev + 1;
+      } else {
+        a.ev = nullptr;
+      }
+      hipError_t e = launch_one(x->prec, a, x->stream);
+      if (e != hipSuccess) return fail(SW2D_E_DEVICE, std::string("launch: ") + hipGetErrorString(e));
     }
-    hipError_t e = x->prec == SW2D_FP64 ? sw2d::launch_iteration_f64(a, x->stream)
-                                        : sw2d::launch_iteration_f32(a, x->stream);
-    if (e != hipSuccess) return fail(SW2D_E_DEVICE, std::string("launch: ") + hipGetErrorString(e));
   }
-  HIPCHK(sw2d::launch_control(x->d_ctrl, 1, 1, x->stream));
+  HIPCHK(sw2d::launch_control(x->d_ctrl_list, 1, 1, x->stream));
   return SW2D_OK;
 }
 
@@ -698,10 +567,10 @@ int sw2d_get_stats(sw2d_ctx *x, sw2d_stats *st) {
     vol.add(part[2 * b]);
     vol.add(part[2 * b + 1]);
   }
-  std::vector<double> acc((size_t)x->n_tiles * 2 * sw2d::ACC_N);
+  std::vector<double> acc((size_t)x->n_acc * 2 * sw2d::ACC_N);
   HIPCHK(hipMemcpy(acc.data(), x->acc, acc.size() * sizeof(double), hipMemcpyDeviceToHost));
   Comp q[sw2d::ACC_N];
-  for (int t = 0; t < x->n_tiles; t++)
+  for (int t = 0; t < x->n_acc; t++)
     for (int k = 0; k < sw2d::ACC_N; k++) {
       q[k].add(acc[(size_t)t * 2 * sw2d::ACC_N + 2 * k]);
       q[k].add(acc[(size_t)t * 2 * sw2d::ACC_N + 2 * k + 1]);
@@ -735,7 +604,7 @@ constexpr uint32_t SNAP_MAGIC = 0x44325753u;  // "SW2D"
 int sw2d_snapshot(sw2d_ctx *x, void *buf, size_t *size) {
   if (!x || !size) return fail(SW2D_E_INVALID, "null argument");
   const size_t nh = (size_t)x->R * x->C, nu = (size_t)(x->R + 1) * x->C, nv = (size_t)x->R * (x->C + 1);
-  const size_t nacc = (size_t)x->n_tiles * 2 * sw2d::ACC_N;
+  const size_t nacc = (size_t)x->n_acc * 2 * sw2d::ACC_N;
   const size_t need = sizeof(SnapHeader) + (nh + nu + nv) * x->esz + nacc * sizeof(double);
   if (!buf) {
     *size = need;
@@ -775,7 +644,7 @@ int sw2d_restore(sw2d_ctx *x, const void *buf, size_t size) {
   if (hd.magic != SNAP_MAGIC || hd.prec != x->prec || hd.R != x->R || hd.C != x->C)
     return fail(SW2D_E_INVALID, "snapshot does not match this context");
   const size_t nh = (size_t)x->R * x->C, nu = (size_t)(x->R + 1) * x->C, nv = (size_t)x->R * (x->C + 1);
-  const size_t nacc = (size_t)x->n_tiles * 2 * sw2d::ACC_N;
+  const size_t nacc = (size_t)x->n_acc * 2 * sw2d::ACC_N;
   if (size < sizeof(hd) + (nh + nu + nv) * x->esz + nacc * sizeof(double))
     return fail(SW2D_E_INVALID, "snapshot truncated");
   Ctrl c;
@@ -802,6 +671,7 @@ int sw2d_restore(sw2d_ctx *x, const void *buf, size_t size) {
   c.iter = hd.iter;
   c.done = hd.done;
   c.smax_bits = hd.smax_bits;
+  std::memset(c.smax_slot, 0, sizeof(c.smax_slot));
   c.D = hd.last_D;
   c.error = 0;
   return x->store_ctrl(c);

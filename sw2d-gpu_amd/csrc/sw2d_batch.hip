@@ -1,0 +1,399 @@
+// sw2d_batch.hip -- batched ensembles: many scenarios of uneven size
+// advanced by one launch per stage over a shared tile work-list, with a
+// per-size-bucket tile-height autotuner and HIP-graph replay.
+//
+// Each scenario is a full sw2d_ctx (own arrays, control block, clock, step
+// and end time).  The batch owns a device array of the scenarios' grid
+// descriptors, an array of their control-block pointers and the work-list.
+// Per-cell arithmetic does not depend on the tile a cell falls in or on the
+// other scenarios, so a scenario's h, U, V are bitwise the same alone and in
+// a batch; only the order of the volume-accounting sums changes.
+#include <map>
+
+#include "sw2d_internal.hpp"
+
+using sw2d::Ctrl;
+using sw2d::Grid;
+using sw2d::Tile;
+
+struct sw2d_batch {
+  int prec = 0;
+  hipStream_t stream = nullptr;
+  std::vector<sw2d_ctx *> scen;   // index = scenario id; nullptr once removed
+  std::vector<int> share_of;      // id whose statics are shared, or -1
+  std::vector<int> rows_override; // manual tile height, 0 = bucket choice
+  std::map<int, int> bucket_rows; // autotuner result per size bucket
+  // device
+  void *d_grids = nullptr;
+  Ctrl **d_ctrls = nullptr;
+  Tile *d_work = nullptr;
+  int *d_done = nullptr;
+  long long *d_iters = nullptr;
+  size_t cap_grids = 0, cap_work = 0, cap_ctrls = 0, cap_done = 0, cap_iters = 0;
+  int n_live = 0, n_work = 0, blocks = 0;
+  std::vector<int> live_ids;      // grid index -> scenario id
+  bool dirty = true;
+  GraphCache graph;
+
+  ~sw2d_batch() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    graph.reset();
+    for (auto *c : scen) delete c;
+    for (void *p : {(void *)d_grids, (void *)d_ctrls, (void *)d_work, (void *)d_done, (void *)d_iters})
+      if (p) (void)hipFree(p);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  // default segment rows of a scenario: its single-grid choice
+  int default_rows(const sw2d_ctx *c) const { return c->TH; }
+  static int bucket_of(const sw2d_ctx *c) {
+    const double cells = (double)c->R * c->C;
+    return (int)std::floor(std::log2(std::max(cells, 1.0)) / 2.0);
+  }
+  int rows_for(int id) const {
+    if (rows_override[id]) return rows_override[id];
+    auto it = bucket_rows.find(bucket_of(scen[id]));
+    return it != bucket_rows.end() ? it->second : default_rows(scen[id]);
+  }
+
+  int grow(void **p, size_t &cap, size_t need) {
+    if (need <= cap) return SW2D_OK;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    size_t n = std::max(need, cap * 2);
+    HIPCHK(hipMalloc(p, n));
+    cap = n;
+    return SW2D_OK;
+  }
+
+  // Tiles of the given scenarios (grid indices), siblings that share
+  // statics interleaved tile by tile, then spread over the 8 XCDs so each
+  // XCD walks a contiguous run of tiles.
+  void build_tiles(const std::vector<int> &gidx, const std::vector<int> &rows, std::vector<Tile> &out) const {
+    std::vector<Tile> list;
+    std::vector<char> used(gidx.size(), 0);
+    for (size_t a = 0; a < gidx.size(); a++) {
+      if (used[a]) continue;
+      std::vector<size_t> group{a};
+      const int ida = live_ids[gidx[a]];
+      for (size_t b2 = a + 1; b2 < gidx.size(); b2++) {
+        const int idb = live_ids[gidx[b2]];
+        if (!used[b2] && share_of[idb] >= 0 &&
+            (share_of[idb] == ida || share_of[idb] == share_of[ida]) && rows[b2] == rows[a])
+          group.push_back(b2);
+      }
+      for (size_t m : group) used[m] = 1;
+      const sw2d_ctx *c = scen[ida];
+      const int th = rows[a];
+      int slot = 0;
+      for (int i0 = 0; i0 < c->R; i0 += th)
+        for (int j0 = 0; j0 < c->C; j0 += sw2d::TILE_W, slot++)
+          for (size_t m : group) list.push_back(Tile{gidx[m], i0, j0, th, slot, 0, 0, 0});
+    }
+    const size_t n = list.size(), per = (n + 7) / 8;
+    out.assign(per * 8, Tile{-1, 0, 0, 0, 0, 0, 0, 0});
+    for (size_t k = 0; k < n; k++) {
+      const size_t x = k / per, r = k % per;
+      out[r * 8 + x] = list[k];
+    }
+  }
+
+  int rebuild() {
+    live_ids.clear();
+    for (size_t id = 0; id < scen.size(); id++)
+      if (scen[id]) live_ids.push_back((int)id);
+    n_live = (int)live_ids.size();
+    const size_t gsz = prec == SW2D_FP64 ? sizeof(Grid<double>) : sizeof(Grid<float>);
+    const size_t nl = std::max<size_t>(1, n_live);
+    int rc = grow(&d_grids, cap_grids, nl * gsz);
+    if (!rc) rc = grow((void **)&d_ctrls, cap_ctrls, nl * sizeof(Ctrl *));
+    if (!rc) rc = grow((void **)&d_done, cap_done, nl * sizeof(int));
+    if (!rc) rc = grow((void **)&d_iters, cap_iters, nl * sizeof(long long));
+    if (rc) return rc;
+    std::vector<unsigned char> grids(std::max<size_t>(1, n_live) * gsz);
+    std::vector<Ctrl *> ctrls(std::max(1, n_live));
+    for (int g = 0; g < n_live; g++) {
+      sw2d_ctx *c = scen[live_ids[g]];
+      HIPCHK(hipMemcpy(grids.data() + g * gsz, c->d_grid, gsz, hipMemcpyDeviceToHost));
+      ctrls[g] = c->d_ctrl;
+    }
+    HIPCHK(hipMemcpy(d_grids, grids.data(), grids.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_ctrls, ctrls.data(), ctrls.size() * sizeof(Ctrl *), hipMemcpyHostToDevice));
+    std::vector<int> gidx(n_live), rows(n_live);
+    for (int g = 0; g < n_live; g++) {
+      gidx[g] = g;
+      rows[g] = rows_for(live_ids[g]);
+    }
+    std::vector<Tile> work;
+    build_tiles(gidx, rows, work);
+    rc = grow((void **)&d_work, cap_work, std::max<size_t>(1, work.size()) * sizeof(Tile));
+    if (rc) return rc;
+    if (!work.empty()) HIPCHK(hipMemcpy(d_work, work.data(), work.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    n_work = (int)work.size();
+    // persistent grid: a multiple of 8 workgroups, at most 8 per CU
+    blocks = std::min(n_work, 2048);
+    graph.reset();
+    dirty = false;
+    return SW2D_OK;
+  }
+
+  sw2d::IterLaunch launch_args() const {
+    sw2d::IterLaunch a;
+    a.grids = d_grids;
+    a.work = d_work;
+    a.n_work = n_work;
+    a.blocks = std::max(1, blocks);
+    a.ctrl = d_ctrls;
+    a.n_ctrl = n_live;
+    a.ev = nullptr;
+    return a;
+  }
+};
+
+extern "C" {
+
+int sw2d_batch_create(int precision, sw2d_batch **out) {
+  if (!out || (precision != SW2D_FP64 && precision != SW2D_FP32)) return fail(SW2D_E_INVALID, "batch precision");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(SW2D_E_DEVICE, "no HIP device available");
+  auto *b = new sw2d_batch();
+  b->prec = precision;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    return fail(SW2D_E_DEVICE, "hipStreamCreate failed");
+  }
+  *out = b;
+  return SW2D_OK;
+}
+
+void sw2d_batch_destroy(sw2d_batch *b) { delete b; }
+
+int sw2d_batch_add(sw2d_batch *b, const sw2d_grid *grid, const sw2d_params *params, int share_with, int *id_out) {
+  if (!b || !grid || !params || !id_out) return fail(SW2D_E_INVALID, "null argument");
+  if (params->precision != b->prec) return fail(SW2D_E_INVALID, "scenario precision differs from the batch");
+  sw2d_ctx *sib = nullptr;
+  if (share_with >= 0) {
+    if (share_with >= (int)b->scen.size() || !(sib = b->scen[share_with]))
+      return fail(SW2D_E_INVALID, "share_with names no live scenario");
+    if (sib->R != grid->rows || sib->C != grid->cols || (grid->n_cell != nullptr) != (sib->ncell != nullptr))
+      return fail(SW2D_E_INVALID, "share_with scenario has another grid shape");
+  }
+  sw2d_ctx *c = nullptr;
+  int rc = sw2d_create(grid, params, &c);
+  if (rc) return rc;
+  if (sib) {
+    // point the descriptor at the sibling's bed and raster, free our copies
+    auto patch = [&](auto &G) -> int {
+      using T = std::remove_pointer_t<decltype(G.Us)>;
+      HIPCHK(hipMemcpy(&G, c->d_grid, sizeof(G), hipMemcpyDeviceToHost));
+      G.z = (const T *)sib->z;
+      G.ncell = (const T *)sib->ncell;
+      G.datum = sib->datum;
+      HIPCHK(hipMemcpy(c->d_grid, &G, sizeof(G), hipMemcpyHostToDevice));
+      return SW2D_OK;
+    };
+    if (b->prec == SW2D_FP64) {
+      Grid<double> G;
+      rc = patch(G);
+    } else {
+      Grid<float> G;
+      rc = patch(G);
+    }
+    if (rc) {
+      delete c;
+      return rc;
+    }
+    c->release(c->z);
+    c->release(c->ncell);
+    c->release(c->datum);
+    c->z = sib->z;
+    c->ncell = sib->ncell;
+    c->datum = sib->datum;
+  }
+  b->scen.push_back(c);
+  b->share_of.push_back(sib ? (b->share_of[share_with] >= 0 ? b->share_of[share_with] : share_with) : -1);
+  b->rows_override.push_back(0);
+  b->dirty = true;
+  *id_out = (int)b->scen.size() - 1;
+  return SW2D_OK;
+}
+
+int sw2d_batch_scenario(sw2d_batch *b, int id, sw2d_ctx **ctx) {
+  if (!b || !ctx || id < 0 || id >= (int)b->scen.size() || !b->scen[id]) return fail(SW2D_E_INVALID, "scenario id");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  *ctx = b->scen[id];
+  return SW2D_OK;
+}
+
+int sw2d_batch_touch(sw2d_batch *b) {
+  if (!b) return fail(SW2D_E_INVALID, "null batch");
+  b->dirty = true;
+  return SW2D_OK;
+}
+
+int sw2d_batch_remove(sw2d_batch *b, int id) {
+  if (!b || id < 0 || id >= (int)b->scen.size() || !b->scen[id]) return fail(SW2D_E_INVALID, "scenario id");
+  for (size_t k = 0; k < b->scen.size(); k++)
+    if (b->scen[k] && (int)k != id && (b->share_of[k] == id))
+      return fail(SW2D_E_STATE, "another scenario shares this scenario's statics");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  sw2d_ctx *c = b->scen[id];
+  if (b->share_of[id] >= 0) {  // statics belong to the sibling: do not free them
+    auto drop = [&](void *p) {
+      auto it = std::find(c->allocs.begin(), c->allocs.end(), p);
+      if (it != c->allocs.end()) c->allocs.erase(it);
+    };
+    drop(c->z);
+    drop(c->ncell);
+    drop(c->datum);
+  }
+  delete c;
+  b->scen[id] = nullptr;
+  b->dirty = true;
+  return SW2D_OK;
+}
+
+int sw2d_batch_set_tile_rows(sw2d_batch *b, int id, int rows) {
+  if (!b || id < 0 || id >= (int)b->scen.size() || !b->scen[id]) return fail(SW2D_E_INVALID, "scenario id");
+  if (rows != 0 && (rows < 4 || rows % 4 != 0 || rows > 4096))
+    return fail(SW2D_E_INVALID, "segment rows must be a multiple of 4 in [4, 4096], or 0");
+  b->rows_override[id] = rows;
+  b->dirty = true;
+  return SW2D_OK;
+}
+
+int sw2d_batch_tile_rows(sw2d_batch *b, int id, int *rows) {
+  if (!b || !rows || id < 0 || id >= (int)b->scen.size() || !b->scen[id]) return fail(SW2D_E_INVALID, "scenario id");
+  *rows = b->rows_for(id);
+  return SW2D_OK;
+}
+
+int sw2d_batch_step(sw2d_batch *b, int64_t n_iter) {
+  if (!b || n_iter < 0) return fail(SW2D_E_INVALID, "step count");
+  if (b->dirty) {
+    int rc = b->rebuild();
+    if (rc) return rc;
+  }
+  if (n_iter == 0 || b->n_live == 0) return SW2D_OK;
+  int rc = enqueue_iterations(b->prec, b->stream, b->launch_args(), n_iter, &b->graph);
+  if (rc) return rc;
+  HIPCHK(sw2d::launch_control(b->d_ctrls, b->n_live, 1, b->stream));
+  return SW2D_OK;
+}
+
+int sw2d_batch_sync(sw2d_batch *b) {
+  if (!b) return fail(SW2D_E_INVALID, "null batch");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return SW2D_OK;
+}
+
+int sw2d_batch_status(sw2d_batch *b, int *done, int64_t *iters, int cap, int *n_out) {
+  if (!b || !n_out) return fail(SW2D_E_INVALID, "null argument");
+  const int n = (int)b->scen.size();
+  *n_out = n;
+  if (cap < n) return fail(SW2D_E_INVALID, "status buffer too small");
+  if (b->dirty) {
+    int rc = b->rebuild();
+    if (rc) return rc;
+  }
+  std::vector<int> dn(std::max(1, b->n_live));
+  std::vector<long long> it(std::max(1, b->n_live));
+  if (b->n_live > 0) {
+    HIPCHK(sw2d::launch_status(b->d_ctrls, b->n_live, b->d_done, b->d_iters, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    HIPCHK(hipMemcpy(dn.data(), b->d_done, b->n_live * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(it.data(), b->d_iters, b->n_live * sizeof(long long), hipMemcpyDeviceToHost));
+  }
+  for (int k = 0; k < n; k++) {
+    if (done) done[k] = 1;
+    if (iters) iters[k] = 0;
+  }
+  for (int g = 0; g < b->n_live; g++) {
+    if (done) done[b->live_ids[g]] = dn[g];
+    if (iters) iters[b->live_ids[g]] = it[g];
+  }
+  return SW2D_OK;
+}
+
+int sw2d_batch_autotune(sw2d_batch *b, int iters) {
+  if (!b || iters <= 0) return fail(SW2D_E_INVALID, "autotune iterations");
+  if (b->dirty) {
+    int rc = b->rebuild();
+    if (rc) return rc;
+  }
+  HIPCHK(hipStreamSynchronize(b->stream));
+  // largest live scenario per size bucket
+  std::map<int, int> probe;
+  for (int g = 0; g < b->n_live; g++) {
+    const int id = b->live_ids[g];
+    const int bk = sw2d_batch::bucket_of(b->scen[id]);
+    auto it = probe.find(bk);
+    if (it == probe.end() || (long long)b->scen[id]->R * b->scen[id]->C >
+                                 (long long)b->scen[b->live_ids[it->second]]->R * b->scen[b->live_ids[it->second]]->C)
+      probe[bk] = g;
+  }
+  const std::vector<int> cands{8, 32, 128, 512};
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  Tile *d_tw = nullptr;
+  size_t cap_tw = 0;
+  int rc = SW2D_OK;
+  for (auto &pb : probe) {
+    const int g = pb.second, id = b->live_ids[g];
+    sw2d_ctx *c = b->scen[id];
+    size_t snap_size = 0;
+    rc = sw2d_snapshot(c, nullptr, &snap_size);
+    if (rc) break;
+    std::vector<unsigned char> snap(snap_size);
+    rc = sw2d_snapshot(c, snap.data(), &snap_size);
+    if (rc) break;
+    Ctrl **single = nullptr;
+    HIPCHK(hipMalloc((void **)&single, sizeof(Ctrl *)));
+    HIPCHK(hipMemcpy(single, &c->d_ctrl, sizeof(Ctrl *), hipMemcpyHostToDevice));
+    float best = 0.0f;
+    int best_rows = b->default_rows(c);
+    for (int th : cands) {
+      if (th > 4 * c->R + 8 && th != cands.front()) continue;  // longer than the grid
+      std::vector<Tile> work;
+      b->build_tiles({g}, {th}, work);
+      rc = b->grow((void **)&d_tw, cap_tw, work.size() * sizeof(Tile));
+      if (rc) break;
+      HIPCHK(hipMemcpy(d_tw, work.data(), work.size() * sizeof(Tile), hipMemcpyHostToDevice));
+      sw2d::IterLaunch a = b->launch_args();
+      a.work = d_tw;
+      a.n_work = (int)work.size();
+      a.blocks = std::min(a.n_work, 2048);
+      a.ctrl = single;
+      a.n_ctrl = 1;
+      rc = enqueue_iterations(b->prec, b->stream, a, 2, nullptr);  // warm
+      HIPCHK(hipEventRecord(e0, b->stream));
+      if (!rc) rc = enqueue_iterations(b->prec, b->stream, a, iters, nullptr);
+      HIPCHK(hipEventRecord(e1, b->stream));
+      HIPCHK(sw2d::launch_control(single, 1, 1, b->stream));
+      HIPCHK(hipStreamSynchronize(b->stream));
+      if (rc) break;
+      float ms = 0.0f;
+      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      if (best == 0.0f || ms < best) {
+        best = ms;
+        best_rows = th;
+      }
+      int rr = sw2d_restore(c, snap.data(), snap.size());
+      if (rr) {
+        rc = rr;
+        break;
+      }
+    }
+    (void)hipFree(single);
+    if (rc) break;
+    b->bucket_rows[pb.first] = best_rows;
+  }
+  if (d_tw) (void)hipFree(d_tw);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  b->dirty = true;
+  return rc;
+}
+
+}  // extern "C"

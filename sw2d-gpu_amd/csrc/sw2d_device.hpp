@@ -22,6 +22,7 @@ constexpr int BLOCK = 256;          // threads per workgroup (4 wavefronts)
 constexpr int MAX_TILE_SRC = 64;    // point sources inside one tile region
 constexpr int DATUM_BLOCK = 64;     // FP32 bed datum block (cells)
 constexpr int ACC_N = 5;            // rain, source, evap, outflow, clamp
+constexpr int SMAX_SLOTS = 64;      // CFL maximum: workgroups spread their atomics
 constexpr int EDGE_N = 1, EDGE_S = 2, EDGE_W = 4, EDGE_E = 8;
 constexpr int ERR_NONFINITE = 1, ERR_TOO_MANY_SRC = 2;
 
@@ -43,7 +44,8 @@ struct Ctrl {
   int run;             // 1 while an iteration is pending
   int done;            // end condition reached
   int error;           // ERR_* bits
-  unsigned long long smax_bits;  // CFL speed max of the newest state
+  unsigned long long smax_bits;  // CFL speed max of the newest state (folded)
+  unsigned long long smax_slot[SMAX_SLOTS];  // per-workgroup atomics land here
   // forcing tables (device pointers)
   const double *rain; int n_rain; double dtrain;
   const double *evapo; int n_evapo; int evaporation_on;
@@ -58,13 +60,22 @@ struct Ctrl {
   double dxdy;
 };
 
+// One work item of the stencil kernels: rows [i0, i0+th) of the 64-column
+// strip starting at column j0 of grid `gid`; its volume accumulators live in
+// slot `slot` of that grid.
+struct Tile {
+  int gid, i0, j0, th, slot, pad0, pad1, pad2;
+};
+
 template <typename T>
 struct Grid {
   int R, C, P;
-  int tiles_x, tiles_y;
+  int tiles_x, tiles_y;          // 64-column strips, row segments
+  int seg_rows;                  // rows per segment when no work-list is given
   int edge_open;
   int has_ncell;
   int rain_block, rain_cols;     // rain multiplier map geometry (0 = none)
+  int rain_shift;                // log2(rain_block)
   int datum_cols;                // FP32: datum blocks per row
   double dx, dy, g, th, manning;
   double INT, INF, LWL, EVmin;

@@ -4,17 +4,17 @@
 
 namespace sw2d {
 
-hipError_t launch_control(Ctrl *ctrl, int n, int commit_only, hipStream_t s);
+hipError_t launch_control(Ctrl *const *ctrl, int n, int commit_only, hipStream_t s);
 
 hipError_t launch_iteration_f32(const IterLaunch &a, hipStream_t s) {
   hipError_t e = launch_control(a.ctrl, a.n_ctrl, 0, s);
   if (e != hipSuccess) return e;
   const Grid<float> *g = static_cast<const Grid<float> *>(a.grids);
   if (a.ev) (void)hipEventRecord(a.ev[0], s);
-  hipLaunchKernelGGL((k_momentum<float, TILE_H_F32>), dim3(a.blocks), dim3(BLOCK), 0, s, g,
+  hipLaunchKernelGGL(k_momentum<float>, dim3(a.blocks), dim3(BLOCK), 0, s, g,
                      a.work, a.n_work);
   if (a.ev) (void)hipEventRecord(a.ev[1], s);
-  hipLaunchKernelGGL((k_continuity<float, TILE_H_F32>), dim3(a.blocks), dim3(BLOCK), 0, s, g,
+  hipLaunchKernelGGL(k_continuity<float>, dim3(a.blocks), dim3(BLOCK), 0, s, g,
                      a.work, a.n_work);
   if (a.ev) (void)hipEventRecord(a.ev[2], s);
   return hipGetLastError();

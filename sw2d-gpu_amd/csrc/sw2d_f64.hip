@@ -6,8 +6,13 @@
 
 namespace sw2d {
 
-hipError_t launch_control(Ctrl *ctrl, int n, int commit_only, hipStream_t s) {
+hipError_t launch_control(Ctrl *const *ctrl, int n, int commit_only, hipStream_t s) {
   hipLaunchKernelGGL(k_control, dim3((n + 63) / 64), dim3(64), 0, s, ctrl, n, commit_only);
+  return hipGetLastError();
+}
+
+hipError_t launch_status(Ctrl *const *ctrl, int n, int *done, long long *iters, hipStream_t s) {
+  hipLaunchKernelGGL(k_status, dim3((n + 63) / 64), dim3(64), 0, s, ctrl, n, done, iters);
   return hipGetLastError();
 }
 
@@ -15,10 +20,10 @@ hipError_t launch_iteration_f64(const IterLaunch &a, hipStream_t s) {
   hipLaunchKernelGGL(k_control, dim3((a.n_ctrl + 63) / 64), dim3(64), 0, s, a.ctrl, a.n_ctrl, 0);
   const Grid<double> *g = static_cast<const Grid<double> *>(a.grids);
   if (a.ev) (void)hipEventRecord(a.ev[0], s);
-  hipLaunchKernelGGL((k_momentum<double, TILE_H_F64>), dim3(a.blocks), dim3(BLOCK), 0, s, g,
+  hipLaunchKernelGGL(k_momentum<double>, dim3(a.blocks), dim3(BLOCK), 0, s, g,
                      a.work, a.n_work);
   if (a.ev) (void)hipEventRecord(a.ev[1], s);
-  hipLaunchKernelGGL((k_continuity<double, TILE_H_F64>), dim3(a.blocks), dim3(BLOCK), 0, s, g,
+  hipLaunchKernelGGL(k_continuity<double>, dim3(a.blocks), dim3(BLOCK), 0, s, g,
                      a.work, a.n_work);
   if (a.ev) (void)hipEventRecord(a.ev[2], s);
   return hipGetLastError();

@@ -1,0 +1,210 @@
+// sw2d_internal.hpp -- host-side context of one scenario, shared by the
+// single-grid runtime (sw2d_api.hip) and the batch/serving runtime
+// (sw2d_batch.hip).  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "sw2d.h"
+#include "sw2d_launch.hpp"
+
+using sw2d::Ctrl;
+using sw2d::Grid;
+
+// message slot behind sw2d_last_error (defined in sw2d_api.hip)
+void sw2d_set_error_text(const std::string &m);
+
+namespace {
+
+inline int fail(int code, const std::string &msg) {
+  sw2d_set_error_text(msg);
+  return code;
+}
+
+#define HIPCHK(x)                                                                        \
+  do {                                                                                   \
+    hipError_t e_ = (x);                                                                 \
+    if (e_ != hipSuccess) return fail(SW2D_E_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; }
+
+struct Comp {
+  double s = 0, c = 0;
+  void add(double x) {
+    double t = s + x;
+    if (std::fabs(s) >= std::fabs(x)) c += (s - t) + x;
+    else c += (x - t) + s;
+    s = t;
+  }
+  double value() const { return s + c; }
+};
+
+// A captured run of GRAPH_ITERS iterations, replayed with one launch.
+constexpr int GRAPH_ITERS = 32;
+struct GraphCache {
+  hipGraphExec_t exec = nullptr;
+  void reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
+  }
+};
+
+inline hipError_t launch_one(int prec, const sw2d::IterLaunch &a, hipStream_t s) {
+  return prec == SW2D_FP64 ? sw2d::launch_iteration_f64(a, s) : sw2d::launch_iteration_f32(a, s);
+}
+
+// Enqueue n iterations; with a cache, whole chunks replay a captured graph
+// (the launch parameters are fixed: everything that changes per iteration
+// lives in the device control blocks).
+inline int enqueue_iterations(int prec, hipStream_t s, const sw2d::IterLaunch &a, int64_t n,
+                              GraphCache *gc) {
+  int64_t k = 0;
+  if (gc && n >= GRAPH_ITERS && !std::getenv("SW2D_NO_GRAPH")) {
+    if (!gc->exec) {
+      hipGraph_t g = nullptr;
+      HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      hipError_t le = hipSuccess;
+      for (int i = 0; i < GRAPH_ITERS && le == hipSuccess; i++) le = launch_one(prec, a, s);
+      hipError_t ce = hipStreamEndCapture(s, &g);
+      if (le != hipSuccess || ce != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(SW2D_E_DEVICE, std::string("graph capture: ") +
+                                       hipGetErrorString(le != hipSuccess ? le : ce));
+      }
+      hipError_t ie = hipGraphInstantiate(&gc->exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ie != hipSuccess) {
+        gc->exec = nullptr;
+        return fail(SW2D_E_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(ie));
+      }
+    }
+    for (; k + GRAPH_ITERS <= n; k += GRAPH_ITERS) HIPCHK(hipGraphLaunch(gc->exec, s));
+  }
+  for (; k < n; k++) {
+    hipError_t e = launch_one(prec, a, s);
+    if (e != hipSuccess) return fail(SW2D_E_DEVICE, std::string("launch: ") + hipGetErrorString(e));
+  }
+  return SW2D_OK;
+}
+
+}  // namespace
+
+struct sw2d_ctx {
+  int prec = 0;
+  size_t esz = 8;
+  int R = 0, C = 0, P = 0, TH = 0, tiles_x = 0, tiles_y = 0, n_tiles = 0;
+  int n_acc = 0;  // accumulator slots: enough for the smallest tile height (4 rows)
+  sw2d_params prm{};
+  double nodata = 0;
+  double dx = 0, dy = 0;
+  hipStream_t stream = nullptr;
+  std::vector<void *> allocs;
+  void *h[2]{}, *U[2]{}, *V[2]{}, *Us = nullptr, *Vs = nullptr, *z = nullptr, *ncell = nullptr;
+  float *datum = nullptr;
+  double *rain_map = nullptr, *acc = nullptr, *partial = nullptr;
+  double *d_uu = nullptr, *d_vv = nullptr;
+  Ctrl *d_ctrl = nullptr;
+  Ctrl **d_ctrl_list = nullptr;  // one-entry pointer array for k_control
+  void *d_grid = nullptr;
+  double *d_rain = nullptr, *d_evapo = nullptr, *d_q = nullptr, *d_dsched = nullptr,
+         *d_dthist = nullptr, *d_ql = nullptr;
+  int *d_src_col = nullptr;
+  long long *d_src_cell = nullptr;
+  int vol_blocks = 0;
+  int dthist_cap = 0;
+  std::vector<unsigned char> active;  // rows*cols
+  std::vector<float> datum_host;
+  int datum_cols = 0;
+  // kernel timing: 4 events per iteration (before control, after control,
+  // after momentum, after continuity)
+  std::vector<hipEvent_t> prof_ev;
+  int prof_cap = 0, prof_used = 0;
+
+  GraphCache graph;
+
+  ~sw2d_ctx() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    graph.reset();
+    for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
+    for (void *p : allocs) (void)hipFree(p);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  int alloc(void **p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes ? bytes : 8);
+    if (e != hipSuccess) {
+      *p = nullptr;
+      return fail(SW2D_E_MEMORY, std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    }
+    allocs.push_back(*p);
+    return SW2D_OK;
+  }
+  void release(void *p) {
+    if (!p) return;
+    auto it = std::find(allocs.begin(), allocs.end(), p);
+    if (it != allocs.end()) allocs.erase(it);
+    (void)hipFree(p);
+  }
+
+  int fetch_ctrl(Ctrl &c) {
+    HIPCHK(hipStreamSynchronize(stream));
+    HIPCHK(hipMemcpy(&c, d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+    return SW2D_OK;
+  }
+  int store_ctrl(const Ctrl &c) {
+    HIPCHK(hipMemcpy(d_ctrl, &c, sizeof(Ctrl), hipMemcpyHostToDevice));
+    return SW2D_OK;
+  }
+  int blocks_for(long long n) const {
+    long long b = (n + sw2d::BLOCK - 1) / sw2d::BLOCK;
+    return (int)std::max(1LL, std::min(b, 4096LL));
+  }
+  int recompute_cfl() {
+    Ctrl c;
+    int rc = fetch_ctrl(c);
+    if (rc) return rc;
+    c.smax_bits = 0;
+    std::memset(c.smax_slot, 0, sizeof(c.smax_slot));
+    rc = store_ctrl(c);
+    if (rc) return rc;
+    HIPCHK(sw2d::launch_cfl_state(prec, d_grid, 0, blocks_for((long long)R * C), stream));
+    HIPCHK(sw2d::launch_control(d_ctrl_list, 1, 1, stream));  // fold the slots
+    HIPCHK(hipStreamSynchronize(stream));
+    return SW2D_OK;
+  }
+  // copy a device array with pitch P into a dense host double array
+  int read_pitched(const void *src, int rows, int cols, double *out) {
+    std::vector<unsigned char> buf((size_t)rows * P * esz);
+    HIPCHK(hipMemcpy(buf.data(), src, buf.size(), hipMemcpyDeviceToHost));
+    for (int i = 0; i < rows; i++)
+      for (int j = 0; j < cols; j++) {
+        size_t k = (size_t)i * P + j;
+        out[(size_t)i * cols + j] = prec == SW2D_FP64 ? ((const double *)buf.data())[k]
+                                                      : (double)((const float *)buf.data())[k];
+      }
+    return SW2D_OK;
+  }
+  int write_pitched(void *dst, int rows, int cols, const double *in, bool cells) {
+    std::vector<unsigned char> buf((size_t)rows * P * esz, 0);
+    for (int i = 0; i < rows; i++)
+      for (int j = 0; j < cols; j++) {
+        double v = in[(size_t)i * cols + j];
+        if (cells && !active[(size_t)i * C + j]) v = -1.0;
+        size_t k = (size_t)i * P + j;
+        if (prec == SW2D_FP64) ((double *)buf.data())[k] = v;
+        else ((float *)buf.data())[k] = (float)v;
+      }
+    HIPCHK(hipMemcpy(dst, buf.data(), buf.size(), hipMemcpyHostToDevice));
+    return SW2D_OK;
+  }
+};
+

@@ -8,21 +8,27 @@
 
 namespace sw2d {
 
-// tile height (rows) of each precision's kernels; tiles are 64 columns wide
-constexpr int TILE_H_F64 = 8;
-constexpr int TILE_H_F32 = 16;
+// default rows per strip segment: the segment length that gives about
+// TARGET_ITEMS work items (8 workgroups per CU), at least 8 rows
+constexpr int TARGET_ITEMS = 2048;
+inline int default_seg_rows(int rows, int strips) {
+  long long s = ((long long)rows * strips + TARGET_ITEMS - 1) / TARGET_ITEMS;
+  s = (s + 3) / 4 * 4;
+  return (int)(s < 8 ? 8 : (s > rows ? ((rows + 3) / 4 * 4) : s));
+}
 
 struct IterLaunch {
   const void *grids;   // device Grid<T> array
-  const int2 *work;    // (grid, tile) work-list, or nullptr for grid 0
+  const Tile *work;    // tile work-list, or nullptr: all THMAX-row tiles of grid 0
   int n_work;          // work items (tiles)
   int blocks;          // workgroups launched (<= n_work; persistent loop)
-  Ctrl *ctrl;          // device control blocks
+  Ctrl *const *ctrl;   // device array of control-block pointers
   int n_ctrl;
   const hipEvent_t *ev;  // optional: recorded after control, momentum, continuity
 };
 
-hipError_t launch_control(Ctrl *ctrl, int n, int commit_only, hipStream_t s);
+hipError_t launch_control(Ctrl *const *ctrl, int n, int commit_only, hipStream_t s);
+hipError_t launch_status(Ctrl *const *ctrl, int n, int *done, long long *iters, hipStream_t s);
 hipError_t launch_iteration_f64(const IterLaunch &a, hipStream_t s);
 hipError_t launch_iteration_f32(const IterLaunch &a, hipStream_t s);
 hipError_t launch_cfl_state(int precision, const void *grids, int gid, int blocks, hipStream_t s);

@@ -30,7 +30,10 @@ EXPORTS = [
     "sw2d_case_source_cells", "sw2d_case_create", "sw2d_write_vtk",
     "sw2d_run_db", "sw2d_profile_begin", "sw2d_profile_end",
     "sw2d_gen_fractal_dem", "sw2d_gen_tilt", "sw2d_gen_uniform",
-    "sw2d_gen_lognormal", "sw2d_gen_gamma",
+    "sw2d_gen_lognormal", "sw2d_gen_gamma", "sw2d_batch_create", "sw2d_batch_destroy",
+    "sw2d_batch_add", "sw2d_batch_scenario", "sw2d_batch_touch", "sw2d_batch_remove",
+    "sw2d_batch_set_tile_rows", "sw2d_batch_autotune", "sw2d_batch_tile_rows",
+    "sw2d_batch_step", "sw2d_batch_sync", "sw2d_batch_status",
 ]
 
 
@@ -120,6 +123,19 @@ def lib():
         L.sw2d_gen_tilt.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, dp]
         for f in ("sw2d_gen_uniform", "sw2d_gen_lognormal", "sw2d_gen_gamma"):
             getattr(L, f).argtypes = [C.c_size_t, C.c_uint64, C.c_double, C.c_double, dp]
+        ip = C.POINTER(C.c_int)
+        L.sw2d_batch_create.argtypes = [C.c_int, C.POINTER(vp)]
+        L.sw2d_batch_destroy.argtypes = [vp]
+        L.sw2d_batch_add.argtypes = [vp, C.POINTER(Grid), C.POINTER(Params), C.c_int, ip]
+        L.sw2d_batch_scenario.argtypes = [vp, C.c_int, C.POINTER(vp)]
+        L.sw2d_batch_touch.argtypes = [vp]
+        L.sw2d_batch_remove.argtypes = [vp, C.c_int]
+        L.sw2d_batch_set_tile_rows.argtypes = [vp, C.c_int, C.c_int]
+        L.sw2d_batch_autotune.argtypes = [vp, C.c_int]
+        L.sw2d_batch_tile_rows.argtypes = [vp, C.c_int, ip]
+        L.sw2d_batch_step.argtypes = [vp, C.c_int64]
+        L.sw2d_batch_sync.argtypes = [vp]
+        L.sw2d_batch_status.argtypes = [vp, ip, C.POINTER(C.c_int64), C.c_int, ip]
         _lib = L
     return _lib
 
@@ -166,15 +182,18 @@ class Sim:
         check(lib().sw2d_create(C.byref(g), C.byref(p), C.byref(h)))
         self.h_ = h
 
+    owned = True
+
     @classmethod
-    def from_handle(cls, handle, rows, cols):
+    def from_handle(cls, handle, rows, cols, owned=True):
         s = cls.__new__(cls)
-        s.h_, s.rows, s.cols = handle, rows, cols
+        s.h_, s.rows, s.cols, s.owned = handle, rows, cols, owned
         return s
 
     def close(self):
         if getattr(self, "h_", None):
-            lib().sw2d_destroy(self.h_)
+            if self.owned:
+                lib().sw2d_destroy(self.h_)
             self.h_ = None
 
     def __del__(self):
@@ -349,3 +368,79 @@ def gen_lognormal(shape, seed, mu, sigma):
 
 def gen_gamma(shape, seed, k, theta):
     return _gen("sw2d_gen_gamma", shape, seed, k, theta)
+
+
+class Batch:
+    """Batched ensemble of scenarios (sw2d_batch_* in include/sw2d.h)."""
+
+    def __init__(self, precision=FP32):
+        h = C.c_void_p()
+        check(lib().sw2d_batch_create(precision, C.byref(h)))
+        self.h_ = h
+        self.precision = precision
+        self.shapes = []
+
+    def close(self):
+        if getattr(self, "h_", None):
+            lib().sw2d_batch_destroy(self.h_)
+            self.h_ = None
+
+    def __del__(self):
+        self.close()
+
+    def add(self, z, h0, dx, *, share_with=-1, dy=None, nodata=-9999.0, n_cell=None, **params):
+        z = np.ascontiguousarray(z, dtype=np.float64)
+        h0 = np.ascontiguousarray(h0, dtype=np.float64)
+        g = Grid(z.shape[0], z.shape[1], float(dx), float(dy if dy is not None else dx),
+                 float(nodata), _dp(z), _dp(h0), None)
+        if n_cell is not None:
+            n_cell = np.ascontiguousarray(n_cell, dtype=np.float64)
+            g.n_cell = _dp(n_cell)
+        params.setdefault("precision", self.precision)
+        p = default_params(**params)
+        out = C.c_int()
+        check(lib().sw2d_batch_add(self.h_, C.byref(g), C.byref(p), int(share_with), C.byref(out)))
+        while len(self.shapes) <= out.value:
+            self.shapes.append(None)
+        self.shapes[out.value] = z.shape
+        return out.value
+
+    def scenario(self, sid):
+        """Non-owning handle of one scenario's context."""
+        h = C.c_void_p()
+        check(lib().sw2d_batch_scenario(self.h_, sid, C.byref(h)))
+        r, c = self.shapes[sid]
+        return Sim.from_handle(h, r, c, owned=False)
+
+    def touch(self):
+        check(lib().sw2d_batch_touch(self.h_))
+
+    def remove(self, sid):
+        check(lib().sw2d_batch_remove(self.h_, sid))
+
+    def set_tile_rows(self, sid, rows):
+        check(lib().sw2d_batch_set_tile_rows(self.h_, sid, rows))
+
+    def tile_rows(self, sid):
+        r = C.c_int()
+        check(lib().sw2d_batch_tile_rows(self.h_, sid, C.byref(r)))
+        return r.value
+
+    def autotune(self, iters=20):
+        check(lib().sw2d_batch_autotune(self.h_, iters))
+
+    def step(self, n):
+        check(lib().sw2d_batch_step(self.h_, int(n)))
+
+    def sync(self):
+        check(lib().sw2d_batch_sync(self.h_))
+
+    def status(self):
+        n = len(self.shapes)
+        done = np.zeros(max(n, 1), dtype=np.int32)
+        it = np.zeros(max(n, 1), dtype=np.int64)
+        nn = C.c_int()
+        check(lib().sw2d_batch_status(self.h_, done.ctypes.data_as(C.POINTER(C.c_int)),
+                                      it.ctypes.data_as(C.POINTER(C.c_int64)), max(n, 1),
+                                      C.byref(nn)))
+        return done[:nn.value], it[:nn.value]

