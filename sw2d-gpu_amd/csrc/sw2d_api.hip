@@ -84,7 +84,7 @@ int sw2d_create(const sw2d_grid *grid, const sw2d_params *params, sw2d_ctx **out
   x->R = R;
   x->C = C;
   x->P = (int)round_up((size_t)C + 1, 64);
-  x->tiles_x = (C + sw2d::TILE_W - 1) / sw2d::TILE_W;
+  x->tiles_x = (C + sw2d::STRIP_W - 1) / sw2d::STRIP_W;
   x->TH = sw2d::default_seg_rows(R, x->tiles_x);
   x->tiles_y = (R + x->TH - 1) / x->TH;
   x->n_tiles = x->tiles_x * x->tiles_y;

@@ -87,7 +87,7 @@ struct sw2d_batch {
       const int th = rows[a];
       int slot = 0;
       for (int i0 = 0; i0 < c->R; i0 += th)
-        for (int j0 = 0; j0 < c->C; j0 += sw2d::TILE_W, slot++)
+        for (int j0 = 0; j0 < c->C; j0 += sw2d::STRIP_W, slot++)
           for (size_t m : group) list.push_back(Tile{gidx[m], i0, j0, th, slot, 0, 0, 0});
     }
     const size_t n = list.size(), per = (n + 7) / 8;
@@ -130,8 +130,8 @@ struct sw2d_batch {
     if (rc) return rc;
     if (!work.empty()) HIPCHK(hipMemcpy(d_work, work.data(), work.size() * sizeof(Tile), hipMemcpyHostToDevice));
     n_work = (int)work.size();
-    // persistent grid: a multiple of 8 workgroups, at most 8 per CU
-    blocks = std::min(n_work, 2048);
+    // one single-wavefront workgroup per work item
+    blocks = n_work;
     graph.reset();
     dirty = false;
     return SW2D_OK;
@@ -363,7 +363,7 @@ int sw2d_batch_autotune(sw2d_batch *b, int iters) {
       sw2d::IterLaunch a = b->launch_args();
       a.work = d_tw;
       a.n_work = (int)work.size();
-      a.blocks = std::min(a.n_work, 2048);
+      a.blocks = a.n_work;
       a.ctrl = single;
       a.n_ctrl = 1;
       rc = enqueue_iterations(b->prec, b->stream, a, 2, nullptr);  // warm

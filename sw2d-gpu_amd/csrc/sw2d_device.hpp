@@ -17,8 +17,9 @@
 
 namespace sw2d {
 
-constexpr int TILE_W = 64;          // tile width: one wavefront per tile row
-constexpr int BLOCK = 256;          // threads per workgroup (4 wavefronts)
+constexpr int STRIP_W = 60;         // output columns of a stencil work item
+constexpr int WAVE = 64;            // threads of a stencil workgroup (one wavefront)
+constexpr int BLOCK = 256;          // threads of the whole-grid helper kernels
 constexpr int MAX_TILE_SRC = 64;    // point sources inside one tile region
 constexpr int DATUM_BLOCK = 64;     // FP32 bed datum block (cells)
 constexpr int ACC_N = 5;            // rain, source, evap, outflow, clamp
@@ -60,7 +61,7 @@ struct Ctrl {
   double dxdy;
 };
 
-// One work item of the stencil kernels: rows [i0, i0+th) of the 64-column
+// One work item of the stencil kernels: rows [i0, i0+th) of the STRIP_W-column
 // strip starting at column j0 of grid `gid`; its volume accumulators live in
 // slot `slot` of that grid.
 struct Tile {
@@ -70,7 +71,7 @@ struct Tile {
 template <typename T>
 struct Grid {
   int R, C, P;
-  int tiles_x, tiles_y;          // 64-column strips, row segments
+  int tiles_x, tiles_y;          // STRIP_W-column strips, row segments
   int seg_rows;                  // rows per segment when no work-list is given
   int edge_open;
   int has_ncell;

@@ -3,18 +3,27 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <stdlib.h>
 
 #include "sw2d_device.hpp"
 
 namespace sw2d {
 
-// default rows per strip segment: the segment length that gives about
-// TARGET_ITEMS work items (8 workgroups per CU), at least 8 rows
-constexpr int TARGET_ITEMS = 2048;
+// default rows per strip segment: 32 (a measured optimum on 8192^2 grids:
+// the 3-4 lead-in rows of a segment stay cheap and ~35k single-wavefront
+// items keep the last round of workgroups short), halved down to 8 while
+// the grid would give fewer than MIN_ITEMS items, so small grids still fill
+// the chip.  SW2D_SEG_ROWS in the environment overrides it (a multiple of 4
+// in [4, 4096]).
+constexpr int MIN_ITEMS = 8192;
 inline int default_seg_rows(int rows, int strips) {
-  long long s = ((long long)rows * strips + TARGET_ITEMS - 1) / TARGET_ITEMS;
-  s = (s + 3) / 4 * 4;
-  return (int)(s < 8 ? 8 : (s > rows ? ((rows + 3) / 4 * 4) : s));
+  if (const char *e = getenv("SW2D_SEG_ROWS")) {
+    const int v = atoi(e);
+    if (v >= 4 && v <= 4096 && v % 4 == 0) return v;
+  }
+  int s = 32;
+  while (s > 8 && (long long)strips * ((rows + s - 1) / s) < MIN_ITEMS) s /= 2;
+  return s;
 }
 
 struct IterLaunch {
