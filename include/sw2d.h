@@ -118,10 +118,10 @@ int sw2d_get_dt_history(sw2d_ctx *ctx, double *out, int capacity, int *n_out);
 int sw2d_step(sw2d_ctx *ctx, int64_t n_iter);
 /* Kernel timing (no reference counterpart; the reference times the whole
  * loop with std::clock, L:1251-1252, L:1477).  After sw2d_profile_begin,
- * each sw2d_step records HIP events on the context stream around the three
+ * each sw2d_step records HIP events on the context stream around the two
  * launches of every iteration (up to max_iters iterations);
- * sw2d_profile_end returns the summed milliseconds of [control, momentum,
- * continuity] and the number of iterations timed. */
+ * sw2d_profile_end returns the summed milliseconds of [control kernel,
+ * stencil step kernel, 0 (reserved)] and the number of iterations timed. */
 int sw2d_profile_begin(sw2d_ctx *ctx, int max_iters);
 int sw2d_profile_end(sw2d_ctx *ctx, double ms_out[3], int64_t *n_iter);
 

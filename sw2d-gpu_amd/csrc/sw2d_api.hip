@@ -31,9 +31,10 @@ void sw2d_params_default(sw2d_params *p) {
 size_t sw2d_footprint_bytes(int rows, int cols, int precision, int has_n) {
   const size_t esz = precision == SW2D_FP64 ? 8 : 4;
   const size_t P = round_up((size_t)cols + 1, 64);
-  const size_t tiles = ((size_t)(cols + 63) / 64) * (((size_t)rows + 3) / 4);
+  const size_t tiles = ((size_t)(cols + sw2d::STRIP_W - 1) / sw2d::STRIP_W) * (((size_t)rows + 3) / 4);
   size_t cells = (size_t)rows * P, faces = (size_t)(rows + 1) * P;
-  size_t b = esz * (2 * cells + 2 * faces + 2 * cells + faces + cells + cells);
+  // h, U, V ping-pong pairs and the bed
+  size_t b = esz * (2 * cells + 2 * faces + 2 * cells + cells);
   if (has_n) b += esz * cells;
   b += tiles * 2 * sw2d::ACC_N * sizeof(double);
   b += 4 * (((size_t)rows + 63) / 64) * (((size_t)cols + 63) / 64);
@@ -107,8 +108,6 @@ int sw2d_create(const sw2d_grid *grid, const sw2d_params *params, sw2d_ctx **out
     rc = rc ? rc : x->alloc(&x->U[b], faces * x->esz);
     rc = rc ? rc : x->alloc(&x->V[b], cells * x->esz);
   }
-  rc = rc ? rc : x->alloc(&x->Us, faces * x->esz);
-  rc = rc ? rc : x->alloc(&x->Vs, cells * x->esz);
   rc = rc ? rc : x->alloc(&x->z, cells * x->esz);
   rc = rc ? rc : x->alloc((void **)&x->acc, (size_t)x->n_acc * 2 * sw2d::ACC_N * sizeof(double));
   x->vol_blocks = x->blocks_for((long long)R * C);
@@ -175,7 +174,6 @@ int sw2d_create(const sw2d_grid *grid, const sw2d_params *params, sw2d_ctx **out
       return bad(fail(SW2D_E_DEVICE, "state clear"));
   }
   if (hipMemcpy(x->h[1], x->h[0], cells * x->esz, hipMemcpyDeviceToDevice) != hipSuccess ||
-      hipMemset(x->Us, 0, faces * x->esz) != hipSuccess || hipMemset(x->Vs, 0, cells * x->esz) != hipSuccess ||
       hipMemset(x->acc, 0, (size_t)x->n_acc * 2 * sw2d::ACC_N * sizeof(double)) != hipSuccess)
     return bad(fail(SW2D_E_DEVICE, "state init"));
 
@@ -196,7 +194,7 @@ int sw2d_create(const sw2d_grid *grid, const sw2d_params *params, sw2d_ctx **out
   if (rc) return bad(rc);
 
   auto fill = [&](auto &G) {
-    using T = std::remove_pointer_t<decltype(G.Us)>;
+    using T = std::remove_pointer_t<std::remove_reference_t<decltype(G.h[0])>>;
     G.R = R;
     G.C = C;
     G.P = x->P;
@@ -222,8 +220,6 @@ int sw2d_create(const sw2d_grid *grid, const sw2d_params *params, sw2d_ctx **out
       G.U[b] = (T *)x->U[b];
       G.V[b] = (T *)x->V[b];
     }
-    G.Us = (T *)x->Us;
-    G.Vs = (T *)x->Vs;
     G.z = (const T *)x->z;
     G.ncell = (const T *)x->ncell;
     G.datum = x->datum;
@@ -252,14 +248,14 @@ int sw2d_create(const sw2d_grid *grid, const sw2d_params *params, sw2d_ctx **out
 
 void sw2d_destroy(sw2d_ctx *ctx) { delete ctx; }
 
-static int update_grid_rain(sw2d_ctx *x, double *map, int block, int cols) {
+static int update_grid_rain(sw2d_ctx *x, void *map, int block, int cols) {
   int shift = 0;
   while (block > 0 && (1 << shift) < block) shift++;
   // patch the rain map fields of the device grid descriptor
   if (x->prec == SW2D_FP64) {
     Grid<double> G;
     HIPCHK(hipMemcpy(&G, x->d_grid, sizeof(G), hipMemcpyDeviceToHost));
-    G.rain_map = map;
+    G.rain_map = (const double *)map;
     G.rain_block = block;
     G.rain_cols = cols;
     G.rain_shift = shift;
@@ -267,7 +263,7 @@ static int update_grid_rain(sw2d_ctx *x, double *map, int block, int cols) {
   } else {
     Grid<float> G;
     HIPCHK(hipMemcpy(&G, x->d_grid, sizeof(G), hipMemcpyDeviceToHost));
-    G.rain_map = map;
+    G.rain_map = (const float *)map;
     G.rain_block = block;
     G.rain_cols = cols;
     G.rain_shift = shift;
@@ -301,9 +297,15 @@ int sw2d_set_rain(sw2d_ctx *x, const double *table, int n, double dtrain, const 
   if (block_map) {
     const int br = (x->R + block - 1) / block;
     cols = (x->C + block - 1) / block;
-    rc = x->alloc((void **)&x->rain_map, (size_t)br * cols * sizeof(double));
+    const size_t nm = (size_t)br * cols;
+    rc = x->alloc(&x->rain_map, nm * x->esz);
     if (rc) return rc;
-    HIPCHK(hipMemcpy(x->rain_map, block_map, (size_t)br * cols * sizeof(double), hipMemcpyHostToDevice));
+    if (x->prec == SW2D_FP64) {
+      HIPCHK(hipMemcpy(x->rain_map, block_map, nm * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+      std::vector<float> mf(block_map, block_map + nm);
+      HIPCHK(hipMemcpy(x->rain_map, mf.data(), nm * sizeof(float), hipMemcpyHostToDevice));
+    }
   }
   return update_grid_rain(x, x->rain_map, block_map ? block : 0, cols);
 }

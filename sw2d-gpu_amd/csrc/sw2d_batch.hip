@@ -184,7 +184,7 @@ int sw2d_batch_add(sw2d_batch *b, const sw2d_grid *grid, const sw2d_params *para
   if (sib) {
     // point the descriptor at the sibling's bed and raster, free our copies
     auto patch = [&](auto &G) -> int {
-      using T = std::remove_pointer_t<decltype(G.Us)>;
+      using T = std::remove_pointer_t<std::remove_reference_t<decltype(G.h[0])>>;
       HIPCHK(hipMemcpy(&G, c->d_grid, sizeof(G), hipMemcpyDeviceToHost));
       G.z = (const T *)sib->z;
       G.ncell = (const T *)sib->ncell;

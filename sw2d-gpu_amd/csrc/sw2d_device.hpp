@@ -7,7 +7,6 @@
 //   U[2]      rows+1 x P   discharge on U faces (row i = face north of cell i)
 //   V[2]      rows   x P   discharge on V faces (column j = face west of j,
 //                          column cols = east boundary face)
-//   Us, Vs    momentum-stage output of the current iteration
 //   z         rows   x P   bed (FP32 form: minus a per-64x64-block datum)
 //   ncell     rows   x P   optional Manning raster (FP32 form: g*n^2)
 #pragma once
@@ -17,7 +16,7 @@
 
 namespace sw2d {
 
-constexpr int STRIP_W = 60;         // output columns of a stencil work item
+constexpr int STRIP_W = 57;         // output columns of a stencil work item
 constexpr int WAVE = 64;            // threads of a stencil workgroup (one wavefront)
 constexpr int BLOCK = 256;          // threads of the whole-grid helper kernels
 constexpr int MAX_TILE_SRC = 64;    // point sources inside one tile region
@@ -83,11 +82,10 @@ struct Grid {
   T *h[2];
   T *U[2];
   T *V[2];
-  T *Us, *Vs;
   const T *z;
   const T *ncell;
   const float *datum;            // FP32: datum per 64x64 block
-  const double *rain_map;
+  const T *rain_map;              // rain multiplier per block, in the state precision
   Ctrl *ctrl;
   double *acc;                   // per tile: ACC_N x (sum, compensation)
 };

@@ -11,12 +11,11 @@ hipError_t launch_iteration_f32(const IterLaunch &a, hipStream_t s) {
   if (e != hipSuccess) return e;
   const Grid<float> *g = static_cast<const Grid<float> *>(a.grids);
   if (a.ev) (void)hipEventRecord(a.ev[0], s);
-  hipLaunchKernelGGL(k_momentum<float>, dim3(a.blocks), dim3(WAVE), 0, s, g,
-                     a.work, a.n_work);
-  if (a.ev) (void)hipEventRecord(a.ev[1], s);
-  hipLaunchKernelGGL(k_continuity<float>, dim3(a.blocks), dim3(WAVE), 0, s, g,
-                     a.work, a.n_work);
-  if (a.ev) (void)hipEventRecord(a.ev[2], s);
+  hipLaunchKernelGGL(k_step<float>, dim3(a.blocks), dim3(WAVE), 0, s, g, a.work, a.n_work);
+  if (a.ev) {
+    (void)hipEventRecord(a.ev[1], s);
+    (void)hipEventRecord(a.ev[2], s);
+  }
   return hipGetLastError();
 }
 

@@ -20,12 +20,11 @@ hipError_t launch_iteration_f64(const IterLaunch &a, hipStream_t s) {
   hipLaunchKernelGGL(k_control, dim3((a.n_ctrl + 63) / 64), dim3(64), 0, s, a.ctrl, a.n_ctrl, 0);
   const Grid<double> *g = static_cast<const Grid<double> *>(a.grids);
   if (a.ev) (void)hipEventRecord(a.ev[0], s);
-  hipLaunchKernelGGL(k_momentum<double>, dim3(a.blocks), dim3(WAVE), 0, s, g,
-                     a.work, a.n_work);
-  if (a.ev) (void)hipEventRecord(a.ev[1], s);
-  hipLaunchKernelGGL(k_continuity<double>, dim3(a.blocks), dim3(WAVE), 0, s, g,
-                     a.work, a.n_work);
-  if (a.ev) (void)hipEventRecord(a.ev[2], s);
+  hipLaunchKernelGGL(k_step<double>, dim3(a.blocks), dim3(WAVE), 0, s, g, a.work, a.n_work);
+  if (a.ev) {
+    (void)hipEventRecord(a.ev[1], s);
+    (void)hipEventRecord(a.ev[2], s);
+  }
   return hipGetLastError();
 }
 

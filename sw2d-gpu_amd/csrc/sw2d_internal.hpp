@@ -108,9 +108,10 @@ struct sw2d_ctx {
   double dx = 0, dy = 0;
   hipStream_t stream = nullptr;
   std::vector<void *> allocs;
-  void *h[2]{}, *U[2]{}, *V[2]{}, *Us = nullptr, *Vs = nullptr, *z = nullptr, *ncell = nullptr;
+  void *h[2]{}, *U[2]{}, *V[2]{}, *z = nullptr, *ncell = nullptr;
   float *datum = nullptr;
-  double *rain_map = nullptr, *acc = nullptr, *partial = nullptr;
+  void *rain_map = nullptr;  // rain multiplier map, in the state precision
+  double *acc = nullptr, *partial = nullptr;
   double *d_uu = nullptr, *d_vv = nullptr;
   Ctrl *d_ctrl = nullptr;
   Ctrl **d_ctrl_list = nullptr;  // one-entry pointer array for k_control

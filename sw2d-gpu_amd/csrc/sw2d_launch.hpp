@@ -9,12 +9,12 @@
 
 namespace sw2d {
 
-// default rows per strip segment: 32 (a measured optimum on 8192^2 grids:
-// the 3-4 lead-in rows of a segment stay cheap and ~35k single-wavefront
-// items keep the last round of workgroups short), halved down to 8 while
-// the grid would give fewer than MIN_ITEMS items, so small grids still fill
-// the chip.  SW2D_SEG_ROWS in the environment overrides it (a multiple of 4
-// in [4, 4096]).
+// default rows per strip segment: 32 (measured on 8192^2 grids: 32 and 64
+// rows are within 2 %, 128 and 256 lose 11 % and 26 % to the last round of
+// workgroups), halved to 16 while the grid would give fewer than MIN_ITEMS
+// items, so small grids still fill the chip; a segment pays 7 lead-in steps,
+// so it never gets shorter than 16 rows.  SW2D_SEG_ROWS in the environment
+// overrides it (a multiple of 4 in [4, 4096]).
 constexpr int MIN_ITEMS = 8192;
 inline int default_seg_rows(int rows, int strips) {
   if (const char *e = getenv("SW2D_SEG_ROWS")) {
@@ -22,7 +22,7 @@ inline int default_seg_rows(int rows, int strips) {
     if (v >= 4 && v <= 4096 && v % 4 == 0) return v;
   }
   int s = 32;
-  while (s > 8 && (long long)strips * ((rows + s - 1) / s) < MIN_ITEMS) s /= 2;
+  while (s > 16 && (long long)strips * ((rows + s - 1) / s) < MIN_ITEMS) s /= 2;
   return s;
 }
 
@@ -33,7 +33,7 @@ struct IterLaunch {
   int blocks;          // workgroups launched (<= n_work; persistent loop)
   Ctrl *const *ctrl;   // device array of control-block pointers
   int n_ctrl;
-  const hipEvent_t *ev;  // optional: recorded after control, momentum, continuity
+  const hipEvent_t *ev;  // optional: recorded after control, after the step kernel (twice)
 };
 
 hipError_t launch_control(Ctrl *const *ctrl, int n, int commit_only, hipStream_t s);

@@ -200,10 +200,69 @@ def test_gpu_snapshot_restore_bitwise():
         assert ta["t"] == tb["t"] and ta["iterations"] == tb["iterations"]
 
 
-def test_gpu_fp32_against_fp64():
-    """FP32 state with the FP64 step sequence replayed: depth within 1e-4 m
-    and volume within 1e-6 relative of the FP64 run."""
-    say("case: FP32 vs FP64, catchment 256x256, 600 iterations")
+def _replay_pair(z, h0, dx, n_iter, *, dt, manning, n_cell, edge_open, rain=None, perturb_bed=None):
+    """FP64 run with the CFL step, then FP32 (and optionally FP64 on a
+    perturbed bed) replaying its step sequence; yields every 20 iterations."""
+    def mk(prec, zz=z, **kw):
+        s = sw2d.Sim(zz, h0, dx, dt=dt, manning=manning, n_cell=n_cell, edge_open=edge_open,
+                     forcing_mode=sw2d.FORCING_STEP, precision=prec, **kw)
+        if rain is not None:
+            s.set_rain(*rain)
+        return s
+    a = mk(sw2d.FP64, cfl=0.5)
+    a.record_dt(n_iter + 10)
+    a.step(n_iter)
+    D = a.dt_history(n_iter + 10)
+    runs = [mk(sw2d.FP64), mk(sw2d.FP32)]
+    if perturb_bed is not None:
+        runs.append(mk(sw2d.FP64, zz=z + perturb_bed))
+    for r in runs:
+        r.set_dt_schedule(D)
+    for k in range(n_iter // 20):
+        for r in runs:
+            r.step(20)
+        yield runs
+
+
+def test_gpu_fp32_smooth_deep_water():
+    """FP32 state on a flow with no wet/dry switching (3-4 m of water over a
+    sloping, undulating bed at 40-41 m, a surface hump released at t = 0,
+    Manning raster): depth within 2e-5 m and discharge within 1e-4 of its
+    maximum of the FP64 run over the same step sequence, volume within 1e-6.
+    The discharge bound is the FP32 floor: a 3.5 m depth is stored to
+    ~1.2e-7 m, an error that enters the surface gradient g*h*(D/dx)*d(eta)
+    every step (~1.6e-6 m^2/s here, ~3e-5 after 300 random-walk steps)."""
+    say("case: FP32 vs FP64, deep water 160x200, 300 iterations")
+    rng = np.random.default_rng(62)
+    R, Cc = 160, 200
+    ii, jj = np.meshgrid(np.arange(R), np.arange(Cc), indexing="ij")
+    z = 40.0 + 0.004 * ii + 0.5 * np.sin(2 * np.pi * jj / Cc) * np.cos(2 * np.pi * ii / R)
+    eta = 44.0 + 0.3 * np.exp(-((ii - 60.0) ** 2 + (jj - 80.0) ** 2) / 200.0)
+    ncell = rng.uniform(0.02, 0.05, (R, Cc))
+    for a, b in _replay_pair(z, eta - z, 10.0, 300, dt=2.0, manning=0.03, n_cell=ncell, edge_open=0):
+        pass
+    eh = float(np.abs(a.read(sw2d.F_H) - b.read(sw2d.F_H)).max())
+    eu = float(np.abs(a.read(sw2d.F_U) - b.read(sw2d.F_U)).max())
+    ev = float(np.abs(a.read(sw2d.F_V) - b.read(sw2d.F_V)).max())
+    sa, sb = a.stats(), b.stats()
+    rel = abs(sa["volume"] - sb["volume"]) / sa["volume"]
+    umax = float(max(np.abs(a.read(sw2d.F_U)).max(), np.abs(a.read(sw2d.F_V)).max()))
+    say("max|dh| %.3e  max|dU| %.3e  max|dV| %.3e  rel volume %.3e  max|q| %.3e" % (eh, eu, ev, rel, umax))
+    assert eh <= 2e-5
+    assert max(eu, ev) <= 1e-4 * umax
+    assert rel <= 1e-6
+
+
+def test_gpu_fp32_catchment_within_fp64_conditioning():
+    """Rain on a sloping noisy catchment with wetting and drying.  The dry-bed
+    rules switch discontinuously (a pit that fills to its spill level changes
+    rule within one step), so any perturbation of the state moves single
+    cells by ~1e-4 m: the FP64 scheme itself, run on the same bed moved by
+    +-1e-7 m (the FP32 bed's representation error), deviates that much.  The
+    FP32 run must stay within twice that FP64 deviation (maximum over the
+    run and count of cells off by more than 1e-5 m), agree to 1e-5 m on
+    99.9 % of the cells and conserve the FP64 run's volume to 1e-6."""
+    say("case: FP32 vs FP64 vs FP64 on a perturbed bed, catchment 256x256, 600 iterations")
     rng = np.random.default_rng(61)
     R = Cc = 256
     ii, jj = np.meshgrid(np.arange(R), np.arange(Cc), indexing="ij")
@@ -211,23 +270,25 @@ def test_gpu_fp32_against_fp64():
     h = np.zeros((R, Cc))
     ncell = rng.uniform(0.03, 0.10, (R, Cc))
     table = np.full(40, 50.0 * 300 / 3600.0)
-    a = sw2d.Sim(z, h, 5.0, dt=1.0, manning=0.05, n_cell=ncell, cfl=0.5, edge_open=sw2d.EDGE_SOUTH,
-                 forcing_mode=sw2d.FORCING_STEP, precision=sw2d.FP64)
-    a.set_rain(table, 300.0)
-    a.record_dt(1000)
-    a.step(600)
-    D = a.dt_history(1000)
-    b = sw2d.Sim(z, h, 5.0, dt=1.0, manning=0.05, n_cell=ncell, edge_open=sw2d.EDGE_SOUTH,
-                 forcing_mode=sw2d.FORCING_STEP, precision=sw2d.FP32)
-    b.set_rain(table, 300.0)
-    b.set_dt_schedule(D)
-    b.step(600)
-    ha, hb = a.read(sw2d.F_H), b.read(sw2d.F_H)
-    err = float(np.abs(ha - hb).max())
+    pert = 1e-7 * np.random.default_rng(7).uniform(-1.0, 1.0, (R, Cc))
+    m32 = mp = 0.0
+    n32 = np_ = 0
+    for a, b, p in _replay_pair(z, h, 5.0, 600, dt=1.0, manning=0.05, n_cell=ncell,
+                                edge_open=sw2d.EDGE_SOUTH, rain=(table, 300.0), perturb_bed=pert):
+        ha = a.read(sw2d.F_H)
+        e32, ep = np.abs(b.read(sw2d.F_H) - ha), np.abs(p.read(sw2d.F_H) - ha)
+        m32, mp = max(m32, float(e32.max())), max(mp, float(ep.max()))
+        n32 += int((e32 > 1e-5).sum())
+        np_ += int((ep > 1e-5).sum())
+    p999 = float(np.quantile(e32, 0.999))
     sa, sb = a.stats(), b.stats()
     rel = abs(sa["volume"] - sb["volume"]) / sa["volume"]
-    say("fp32 max|dh| = %.3e  rel volume = %.3e  t = %.3f/%.3f" % (err, rel, sa["t"], sb["t"]))
-    assert err <= 1e-4
+    say("fp32: max %.3e  cells>1e-5 %d  p99.9 %.3e  rel volume %.3e | fp64 bed+-1e-7: max %.3e  "
+        "cells>1e-5 %d" % (m32, n32, p999, rel, mp, np_))
+    assert mp > 1e-5  # the perturbed FP64 run does switch somewhere
+    assert m32 <= 2.0 * mp
+    assert n32 <= 2 * np_ + 30
+    assert p999 <= 1e-5
     assert rel <= 1e-6
 
 
