@@ -72,6 +72,98 @@ def cpu_baseline(w, h, U, V, seconds=15.0, max_iters=40):
                        "(%.1f s)" % (n, w["z"].shape[0], w["z"].shape[1], dt))
 
 
+def ensemble_bench(n_grids, chunk=100):
+    """Config 4: the whole ensemble in one batch (statics shared by the
+    scenarios of a grid), autotuned segment lengths, stepped as replayed
+    HIP graphs until every scenario reached its 2 h end time."""
+    import sw2d
+    import workloads
+    t0 = time.perf_counter()
+    w = workloads.config4(n_grids=n_grids)
+    b, ids, cells = workloads.make_batch(w)
+    log("ensemble: %d scenarios on %d grids (%.1f M cells), built in %.1f s"
+        % (len(ids), n_grids, cells.sum() / 1e6, time.perf_counter() - t0))
+    t1 = time.perf_counter()
+    b.autotune(10)
+    rows = {}
+    for g, sid in zip(w["grids"], ids[::4]):
+        rows.setdefault(str(int(np.log2(g["side"] ** 2) // 2)), b.tile_rows(sid))
+    tune_s = time.perf_counter() - t1
+    b.sync()
+    t2 = time.perf_counter()
+    launches = 0
+    while True:
+        b.step(chunk)
+        launches += chunk
+        done, iters = b.status()
+        if done[ids].all():
+            break
+    wall = time.perf_counter() - t2
+    it = iters[ids]
+    cu = float((cells * it).sum())
+    b.close()
+    res = {"workload": w["desc"], "scenarios": len(ids), "grids": n_grids,
+           "cells_total": int(cells.sum()), "wall_s": round(wall, 3),
+           "scenarios_per_s": round(len(ids) / wall, 3),
+           "gcell_updates_per_s": round(cu / wall / 1e9, 3),
+           "iterations_min_max": [int(it.min()), int(it.max())], "batch_iterations": launches,
+           "autotune_s": round(tune_s, 2), "segment_rows_by_size_bucket": rows}
+    log("ensemble: %s" % res)
+    return res
+
+
+def serving_bench(n_requests, rate, chunk=20):
+    """Config 5: requests submitted at their Poisson arrival times (wall
+    clock) to the serving loop, which admits them by footprint, batches the
+    running ones and retires them; time-to-result = finish - submit."""
+    import sw2d
+    import workloads
+    t0 = time.perf_counter()
+    w = workloads.config5(n_requests=n_requests, rate=rate)
+    reqs = w["requests"]
+    log("serving: %d requests generated in %.1f s" % (len(reqs), time.perf_counter() - t0))
+    budget = 250 * 10 ** 9   # admission budget, below the 288 GB of HBM
+    srv = sw2d.Server(sw2d.FP32, mem_budget=budget)
+    p = dict(w["params"])
+    ids = [None] * len(reqs)
+    nxt, finished = 0, set()
+    cells = np.array([r["z"].size for r in reqs], dtype=np.int64)
+    start = time.perf_counter()
+    while len(finished) < len(reqs):
+        now = time.perf_counter() - start
+        while nxt < len(reqs) and reqs[nxt]["arrival"] <= now:
+            r = reqs[nxt]
+            ids[nxt] = srv.submit(r["z"], np.zeros_like(r["z"]), w["dx"], n_cell=r["n_cell"],
+                                  rain=r["rain"], dtrain=w["dtrain"], t_end=r["duration"], **p)
+            nxt += 1
+        running, queued = srv.advance(chunk if nxt > len(finished) else 0)
+        for k in range(nxt):
+            if k not in finished and srv.info(ids[k])["state"] == sw2d.REQ_DONE:
+                finished.add(k)
+                srv.release(ids[k])
+        if running == 0 and queued == 0 and nxt < len(reqs):
+            time.sleep(max(0.0, reqs[nxt]["arrival"] - (time.perf_counter() - start)))
+    wall = time.perf_counter() - start
+    info = [srv.info(i) for i in ids]
+    srv.close()
+    ttr = np.array([i["finish_s"] - i["submit_s"] for i in info])
+    queue = np.array([i["admit_s"] - i["submit_s"] for i in info])
+    service = np.array([i["finish_s"] - i["admit_s"] for i in info])
+    it = np.array([i["iterations"] for i in info])
+    res = {"workload": w["desc"], "requests": len(reqs), "arrival_rate_per_s": rate,
+           "wall_s": round(wall, 3), "scenarios_per_s": round(len(reqs) / wall, 3),
+           "gcell_updates_per_s": round(float((cells * it).sum()) / wall / 1e9, 3),
+           "time_to_result_s": {"p50": round(float(np.percentile(ttr, 50)), 3),
+                                "p99": round(float(np.percentile(ttr, 99)), 3)},
+           "queue_s": {"p50": round(float(np.percentile(queue, 50)), 3),
+                       "p99": round(float(np.percentile(queue, 99)), 3)},
+           "service_s": {"p50": round(float(np.percentile(service, 50)), 3),
+                         "p99": round(float(np.percentile(service, 99)), 3)},
+           "memory_budget_bytes": budget}
+    log("serving: %s" % res)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,7 +173,24 @@ def main():
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--ensemble-grids", type=int, default=64)
+    ap.add_argument("--serving-requests", type=int, default=48)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 256 MiB device copy 5 times (the reference kernel for "
+                         "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = 256 * 1024 * 1024 // 4
+        a = torch.ones(n, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n * 4, "copies": 5}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU sharding is a design "
               "(row strips with a 3-row halo), not implemented", file=sys.stderr)
@@ -166,6 +275,10 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, h, U, V)
         log("cpu baseline: %s" % out["cpu_baseline"])
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble_bench(args.ensemble_grids)
+    if not args.no_serving:
+        out["serving"] = serving_bench(args.serving_requests, 50.0)
     print(json.dumps(out), flush=True)
 
 

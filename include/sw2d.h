@@ -188,24 +188,72 @@ int sw2d_batch_add(sw2d_batch *b, const sw2d_grid *grid, const sw2d_params *para
 /* The scenario's context (for set_rain/read/stats/snapshot).  Waits for
  * the batch stream.  Changing forcing tables needs sw2d_batch_touch. */
 int sw2d_batch_scenario(sw2d_batch *b, int id, sw2d_ctx **ctx);
+/* Call after changing any scenario through its context (forcing tables,
+ * sw2d_restore, sw2d_write_state): rebuilds the launch descriptors and
+ * returns finished scenarios to the work-list. */
 int sw2d_batch_touch(sw2d_batch *b);
 /* Removes a scenario from the work-list and frees its device memory. */
 int sw2d_batch_remove(sw2d_batch *b, int id);
 /* Segment length (rows) of one scenario's work items: a multiple of 4 in
  * [4, 4096]; 0 = the autotuned choice for its size bucket. */
 int sw2d_batch_set_tile_rows(sw2d_batch *b, int id, int rows);
-/* Times every candidate segment length (8, 32, 128, 512 rows) on the
- * largest scenario of each size bucket (floor(log2(cells)/2)) for `iters`
- * iterations, keeps the fastest per bucket, and restores the probed
- * scenarios bitwise. */
+/* Chooses the segment length of each size bucket (floor(log2(cells)/2)):
+ * with one iteration of every live scenario decided but not committed, the
+ * whole batch's step kernel is timed over `iters` launches for every
+ * candidate (16, 32, 64, 128 rows) of one bucket at a time, largest bucket
+ * first, and the fastest is kept.  Control blocks and volume accumulators
+ * are restored afterwards, so the batch state is bitwise unchanged. */
 int sw2d_batch_autotune(sw2d_batch *b, int iters);
 int sw2d_batch_tile_rows(sw2d_batch *b, int id, int *rows);
 int sw2d_batch_step(sw2d_batch *b, int64_t n_iter);
 int sw2d_batch_sync(sw2d_batch *b);
 /* done[k] = 1 when scenario k (ids 0..n-1, removed ones report 1) has
  * reached its t_end; iters[k] = its committed iterations (either may be
- * NULL).  Synchronises. */
+ * NULL).  Synchronises.  Scenarios seen finished leave the work-list at
+ * the next step (sw2d_batch_touch returns them). */
 int sw2d_batch_status(sw2d_batch *b, int *done, int64_t *iters, int cap, int *n_out);
+
+/* ---- serving loop (no reference counterpart) -----------------------------
+ * Continuous batching of scenario requests on one device.  Requests queue
+ * on submit (the server keeps host copies of their inputs).  Each
+ * sw2d_server_advance admits queued requests in arrival order while their
+ * device footprint (sw2d_footprint_bytes) fits in the memory budget and a
+ * slot is free, steps every admitted scenario n_iter iterations as one
+ * batch (each scenario with its own clock and CFL step), and retires the
+ * scenarios that reached their t_end: their final h, U, V and statistics
+ * are copied to host memory and their device memory is freed for the next
+ * admissions.  A scenario's result equals the same scenario run alone
+ * (sw2d_create + sw2d_step), bitwise in h, U and V. */
+typedef struct sw2d_server sw2d_server;
+#define SW2D_REQ_QUEUED 0
+#define SW2D_REQ_RUNNING 1
+#define SW2D_REQ_DONE 2
+typedef struct {
+  int state;               /* SW2D_REQ_*                                   */
+  int rows, cols;
+  double submit_s;         /* server clock (seconds since create)          */
+  double admit_s;          /* -1 until admitted                            */
+  double finish_s;         /* -1 until retired                             */
+  int64_t iterations;      /* committed iterations (when retired)          */
+} sw2d_request_info;
+/* mem_budget: device bytes the admitted scenarios may hold; max_slots:
+ * concurrent scenarios (0 = no limit). */
+int sw2d_server_create(int precision, size_t mem_budget, int max_slots, sw2d_server **out);
+void sw2d_server_destroy(sw2d_server *s);
+/* Queues a request: grid and parameters as sw2d_create (params->t_end > 0
+ * required), rain table [mm per dtrain] as sw2d_set_rain (n_rain may be 0).
+ * Fails with SW2D_E_MEMORY when the request alone exceeds the budget. */
+int sw2d_server_submit(sw2d_server *s, const sw2d_grid *grid, const sw2d_params *params,
+                       const double *rain_mm, int n_rain, double dtrain, int *req_id);
+/* Admit, step n_iter iterations, retire (synchronises).  Optional outputs:
+ * requests running and queued after the call. */
+int sw2d_server_advance(sw2d_server *s, int64_t n_iter, int *n_running, int *n_queued);
+int sw2d_server_info(sw2d_server *s, int req_id, sw2d_request_info *info);
+/* Final field (SW2D_FIELD_H, _U or _V) or statistics of a retired request. */
+int sw2d_server_read(sw2d_server *s, int req_id, int field, double *host_out);
+int sw2d_server_stats(sw2d_server *s, int req_id, sw2d_stats *out);
+/* Drops a retired request's host result. */
+int sw2d_server_release(sw2d_server *s, int req_id);
 
 /* ---- reference case directories (db/<case>/input, output) ------------ */
 typedef struct sw2d_case sw2d_case;

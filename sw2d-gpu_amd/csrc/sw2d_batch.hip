@@ -8,6 +8,7 @@
 // Per-cell arithmetic does not depend on the tile a cell falls in or on the
 // other scenarios, so a scenario's h, U, V are bitwise the same alone and in
 // a batch; only the order of the volume-accounting sums changes.
+#include <algorithm>
 #include <map>
 
 #include "sw2d_internal.hpp"
@@ -22,6 +23,7 @@ struct sw2d_batch {
   std::vector<sw2d_ctx *> scen;   // index = scenario id; nullptr once removed
   std::vector<int> share_of;      // id whose statics are shared, or -1
   std::vector<int> rows_override; // manual tile height, 0 = bucket choice
+  std::vector<char> parked;       // seen at its end time by status: left out of the work-list
   std::map<int, int> bucket_rows; // autotuner result per size bucket
   // device
   void *d_grids = nullptr;
@@ -119,10 +121,11 @@ struct sw2d_batch {
     }
     HIPCHK(hipMemcpy(d_grids, grids.data(), grids.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(d_ctrls, ctrls.data(), ctrls.size() * sizeof(Ctrl *), hipMemcpyHostToDevice));
-    std::vector<int> gidx(n_live), rows(n_live);
+    std::vector<int> gidx, rows;
     for (int g = 0; g < n_live; g++) {
-      gidx[g] = g;
-      rows[g] = rows_for(live_ids[g]);
+      if (parked[live_ids[g]]) continue;  // finished: its work items would only exit
+      gidx.push_back(g);
+      rows.push_back(rows_for(live_ids[g]));
     }
     std::vector<Tile> work;
     build_tiles(gidx, rows, work);
@@ -213,6 +216,7 @@ int sw2d_batch_add(sw2d_batch *b, const sw2d_grid *grid, const sw2d_params *para
   b->scen.push_back(c);
   b->share_of.push_back(sib ? (b->share_of[share_with] >= 0 ? b->share_of[share_with] : share_with) : -1);
   b->rows_override.push_back(0);
+  b->parked.push_back(0);
   b->dirty = true;
   *id_out = (int)b->scen.size() - 1;
   return SW2D_OK;
@@ -227,6 +231,7 @@ int sw2d_batch_scenario(sw2d_batch *b, int id, sw2d_ctx **ctx) {
 
 int sw2d_batch_touch(sw2d_batch *b) {
   if (!b) return fail(SW2D_E_INVALID, "null batch");
+  std::fill(b->parked.begin(), b->parked.end(), 0);
   b->dirty = true;
   return SW2D_OK;
 }
@@ -311,6 +316,10 @@ int sw2d_batch_status(sw2d_batch *b, int *done, int64_t *iters, int cap, int *n_
   for (int g = 0; g < b->n_live; g++) {
     if (done) done[b->live_ids[g]] = dn[g];
     if (iters) iters[b->live_ids[g]] = it[g];
+    if (dn[g] && !b->parked[b->live_ids[g]]) {  // finished: drop its items at the next step
+      b->parked[b->live_ids[g]] = 1;
+      b->dirty = true;
+    }
   }
   return SW2D_OK;
 }
@@ -321,77 +330,103 @@ int sw2d_batch_autotune(sw2d_batch *b, int iters) {
     int rc = b->rebuild();
     if (rc) return rc;
   }
+  if (b->n_live == 0) return SW2D_OK;
   HIPCHK(hipStreamSynchronize(b->stream));
-  // largest live scenario per size bucket
-  std::map<int, int> probe;
+  // Save every live scenario's control block and volume accumulators, then
+  // decide one iteration for all of them without committing it: the step
+  // kernel then reads the committed buffers and rewrites the scratch ones
+  // the same way on every launch, so candidates are timed on the real mix
+  // and restoring the saved blocks leaves the batch bitwise unchanged.
+  std::vector<Ctrl> ctrl_save(b->n_live);
+  std::vector<size_t> acc_off(b->n_live + 1, 0);
+  for (int g = 0; g < b->n_live; g++)
+    acc_off[g + 1] = acc_off[g] + (size_t)b->scen[b->live_ids[g]]->n_acc * 2 * sw2d::ACC_N;
+  double *d_acc_save = nullptr;
+  HIPCHK(hipMalloc((void **)&d_acc_save, std::max<size_t>(1, acc_off.back()) * sizeof(double)));
+  for (int g = 0; g < b->n_live; g++) {
+    sw2d_ctx *c = b->scen[b->live_ids[g]];
+    HIPCHK(hipMemcpy(&ctrl_save[g], c->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(d_acc_save + acc_off[g], c->acc, (acc_off[g + 1] - acc_off[g]) * sizeof(double),
+                     hipMemcpyDeviceToDevice));
+  }
+  HIPCHK(sw2d::launch_control(b->d_ctrls, b->n_live, 0, b->stream));
+
+  // buckets probed largest total first; scenarios with a manual height keep it
+  std::map<int, long long> cells;
+  std::map<int, int> choice;
   for (int g = 0; g < b->n_live; g++) {
     const int id = b->live_ids[g];
+    if (b->rows_override[id]) continue;
     const int bk = sw2d_batch::bucket_of(b->scen[id]);
-    auto it = probe.find(bk);
-    if (it == probe.end() || (long long)b->scen[id]->R * b->scen[id]->C >
-                                 (long long)b->scen[b->live_ids[it->second]]->R * b->scen[b->live_ids[it->second]]->C)
-      probe[bk] = g;
+    cells[bk] += (long long)b->scen[id]->R * b->scen[id]->C;
+    if (!choice.count(bk)) choice[bk] = b->rows_for(id);
   }
-  const std::vector<int> cands{8, 32, 128, 512};
+  std::vector<std::pair<long long, int>> order;
+  for (auto &kv : cells) order.push_back({-kv.second, kv.first});
+  std::sort(order.begin(), order.end());
+
+  const std::vector<int> cands{16, 32, 64, 128};
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   Tile *d_tw = nullptr;
   size_t cap_tw = 0;
   int rc = SW2D_OK;
-  for (auto &pb : probe) {
-    const int g = pb.second, id = b->live_ids[g];
-    sw2d_ctx *c = b->scen[id];
-    size_t snap_size = 0;
-    rc = sw2d_snapshot(c, nullptr, &snap_size);
-    if (rc) break;
-    std::vector<unsigned char> snap(snap_size);
-    rc = sw2d_snapshot(c, snap.data(), &snap_size);
-    if (rc) break;
-    Ctrl **single = nullptr;
-    HIPCHK(hipMalloc((void **)&single, sizeof(Ctrl *)));
-    HIPCHK(hipMemcpy(single, &c->d_ctrl, sizeof(Ctrl *), hipMemcpyHostToDevice));
+  auto time_choice = [&](float &ms) -> int {
+    std::vector<int> gidx(b->n_live), rows(b->n_live);
+    for (int g = 0; g < b->n_live; g++) {
+      const int id = b->live_ids[g];
+      gidx[g] = g;
+      rows[g] = b->rows_override[id] ? b->rows_override[id] : choice[sw2d_batch::bucket_of(b->scen[id])];
+    }
+    std::vector<Tile> work;
+    b->build_tiles(gidx, rows, work);
+    int r = b->grow((void **)&d_tw, cap_tw, work.size() * sizeof(Tile));
+    if (r) return r;
+    HIPCHK(hipMemcpy(d_tw, work.data(), work.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    sw2d::IterLaunch a = b->launch_args();
+    a.work = d_tw;
+    a.n_work = (int)work.size();
+    a.blocks = a.n_work;
+    HIPCHK(sw2d::launch_step_only(b->prec, a, b->stream));  // warm
+    HIPCHK(hipEventRecord(e0, b->stream));
+    for (int k = 0; k < iters; k++) HIPCHK(sw2d::launch_step_only(b->prec, a, b->stream));
+    HIPCHK(hipEventRecord(e1, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    return SW2D_OK;
+  };
+  for (auto &o : order) {
+    const int bk = o.second;
+    int best_rows = choice[bk];
     float best = 0.0f;
-    int best_rows = b->default_rows(c);
     for (int th : cands) {
-      if (th > 4 * c->R + 8 && th != cands.front()) continue;  // longer than the grid
-      std::vector<Tile> work;
-      b->build_tiles({g}, {th}, work);
-      rc = b->grow((void **)&d_tw, cap_tw, work.size() * sizeof(Tile));
-      if (rc) break;
-      HIPCHK(hipMemcpy(d_tw, work.data(), work.size() * sizeof(Tile), hipMemcpyHostToDevice));
-      sw2d::IterLaunch a = b->launch_args();
-      a.work = d_tw;
-      a.n_work = (int)work.size();
-      a.blocks = a.n_work;
-      a.ctrl = single;
-      a.n_ctrl = 1;
-      rc = enqueue_iterations(b->prec, b->stream, a, 2, nullptr);  // warm
-      HIPCHK(hipEventRecord(e0, b->stream));
-      if (!rc) rc = enqueue_iterations(b->prec, b->stream, a, iters, nullptr);
-      HIPCHK(hipEventRecord(e1, b->stream));
-      HIPCHK(sw2d::launch_control(single, 1, 1, b->stream));
-      HIPCHK(hipStreamSynchronize(b->stream));
-      if (rc) break;
+      choice[bk] = th;
       float ms = 0.0f;
-      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      rc = time_choice(ms);
+      if (rc) break;
       if (best == 0.0f || ms < best) {
         best = ms;
         best_rows = th;
       }
-      int rr = sw2d_restore(c, snap.data(), snap.size());
-      if (rr) {
-        rc = rr;
-        break;
-      }
     }
-    (void)hipFree(single);
     if (rc) break;
-    b->bucket_rows[pb.first] = best_rows;
+    choice[bk] = best_rows;
   }
+  // restore the control blocks and accumulators (also after a failure)
+  for (int g = 0; g < b->n_live; g++) {
+    sw2d_ctx *c = b->scen[b->live_ids[g]];
+    (void)hipMemcpy(c->d_ctrl, &ctrl_save[g], sizeof(Ctrl), hipMemcpyHostToDevice);
+    (void)hipMemcpy(c->acc, d_acc_save + acc_off[g], (acc_off[g + 1] - acc_off[g]) * sizeof(double),
+                    hipMemcpyDeviceToDevice);
+  }
+  (void)hipStreamSynchronize(b->stream);
+  (void)hipFree(d_acc_save);
   if (d_tw) (void)hipFree(d_tw);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  if (!rc)
+    for (auto &kv : choice) b->bucket_rows[kv.first] = kv.second;
   b->dirty = true;
   return rc;
 }

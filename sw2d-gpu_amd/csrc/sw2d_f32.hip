@@ -19,6 +19,12 @@ hipError_t launch_iteration_f32(const IterLaunch &a, hipStream_t s) {
   return hipGetLastError();
 }
 
+hipError_t launch_step_only_f32(const IterLaunch &a, hipStream_t s) {
+  hipLaunchKernelGGL(k_step<float>, dim3(a.blocks), dim3(WAVE), 0, s,
+                     static_cast<const Grid<float> *>(a.grids), a.work, a.n_work);
+  return hipGetLastError();
+}
+
 hipError_t launch_cfl_state_f64(const void *grids, int gid, int blocks, hipStream_t s);
 hipError_t launch_cell_velocity_f64(const void *grids, int gid, double *uu, double *vv, int blocks,
                                     hipStream_t s);

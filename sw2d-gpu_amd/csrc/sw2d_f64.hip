@@ -28,6 +28,13 @@ hipError_t launch_iteration_f64(const IterLaunch &a, hipStream_t s) {
   return hipGetLastError();
 }
 
+hipError_t launch_step_only(int precision, const IterLaunch &a, hipStream_t s) {
+  if (precision != 0) return launch_step_only_f32(a, s);
+  hipLaunchKernelGGL(k_step<double>, dim3(a.blocks), dim3(WAVE), 0, s,
+                     static_cast<const Grid<double> *>(a.grids), a.work, a.n_work);
+  return hipGetLastError();
+}
+
 hipError_t launch_cfl_state_f64(const void *grids, int gid, int blocks, hipStream_t s) {
   hipLaunchKernelGGL(k_cfl_state<double>, dim3(blocks), dim3(BLOCK), 0, s,
                      static_cast<const Grid<double> *>(grids), gid);

@@ -40,6 +40,10 @@ hipError_t launch_control(Ctrl *const *ctrl, int n, int commit_only, hipStream_t
 hipError_t launch_status(Ctrl *const *ctrl, int n, int *done, long long *iters, hipStream_t s);
 hipError_t launch_iteration_f64(const IterLaunch &a, hipStream_t s);
 hipError_t launch_iteration_f32(const IterLaunch &a, hipStream_t s);
+// the stencil step alone (no control kernel): with an iteration decided and
+// not committed, repeated launches recompute the same scratch state
+hipError_t launch_step_only(int precision, const IterLaunch &a, hipStream_t s);
+hipError_t launch_step_only_f32(const IterLaunch &a, hipStream_t s);
 hipError_t launch_cfl_state(int precision, const void *grids, int gid, int blocks, hipStream_t s);
 hipError_t launch_cell_velocity(int precision, const void *grids, int gid, double *uu, double *vv,
                                 int blocks, hipStream_t s);

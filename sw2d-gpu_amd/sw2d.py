@@ -34,7 +34,11 @@ EXPORTS = [
     "sw2d_batch_add", "sw2d_batch_scenario", "sw2d_batch_touch", "sw2d_batch_remove",
     "sw2d_batch_set_tile_rows", "sw2d_batch_autotune", "sw2d_batch_tile_rows",
     "sw2d_batch_step", "sw2d_batch_sync", "sw2d_batch_status",
+    "sw2d_server_create", "sw2d_server_destroy", "sw2d_server_submit", "sw2d_server_advance",
+    "sw2d_server_info", "sw2d_server_read", "sw2d_server_stats", "sw2d_server_release",
 ]
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE = 0, 1, 2
 
 
 class Grid(C.Structure):
@@ -58,6 +62,12 @@ class Stats(C.Structure):
                 ("done", C.c_int), ("volume", C.c_double), ("rain_in", C.c_double),
                 ("source_in", C.c_double), ("evap_out", C.c_double),
                 ("outflow", C.c_double), ("clamp_gain", C.c_double)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("state", C.c_int), ("rows", C.c_int), ("cols", C.c_int),
+                ("submit_s", C.c_double), ("admit_s", C.c_double), ("finish_s", C.c_double),
+                ("iterations", C.c_int64)]
 
 
 class CaseInfo(C.Structure):
@@ -136,6 +146,15 @@ def lib():
         L.sw2d_batch_step.argtypes = [vp, C.c_int64]
         L.sw2d_batch_sync.argtypes = [vp]
         L.sw2d_batch_status.argtypes = [vp, ip, C.POINTER(C.c_int64), C.c_int, ip]
+        L.sw2d_server_create.argtypes = [C.c_int, C.c_size_t, C.c_int, C.POINTER(vp)]
+        L.sw2d_server_destroy.argtypes = [vp]
+        L.sw2d_server_submit.argtypes = [vp, C.POINTER(Grid), C.POINTER(Params), dp, C.c_int,
+                                         C.c_double, ip]
+        L.sw2d_server_advance.argtypes = [vp, C.c_int64, ip, ip]
+        L.sw2d_server_info.argtypes = [vp, C.c_int, C.POINTER(RequestInfo)]
+        L.sw2d_server_read.argtypes = [vp, C.c_int, C.c_int, dp]
+        L.sw2d_server_stats.argtypes = [vp, C.c_int, C.POINTER(Stats)]
+        L.sw2d_server_release.argtypes = [vp, C.c_int]
         _lib = L
     return _lib
 
@@ -444,3 +463,68 @@ class Batch:
                                       it.ctypes.data_as(C.POINTER(C.c_int64)), max(n, 1),
                                       C.byref(nn)))
         return done[:nn.value], it[:nn.value]
+
+
+class Server:
+    """Serving loop with continuous batching (sw2d_server_* in include/sw2d.h)."""
+
+    def __init__(self, precision=FP32, mem_budget=1 << 30, max_slots=0):
+        h = C.c_void_p()
+        check(lib().sw2d_server_create(precision, int(mem_budget), int(max_slots), C.byref(h)))
+        self.h_ = h
+        self.precision = precision
+        self.shapes = []
+
+    def close(self):
+        if getattr(self, "h_", None):
+            lib().sw2d_server_destroy(self.h_)
+            self.h_ = None
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, z, h0, dx, *, rain=None, dtrain=300.0, dy=None, nodata=-9999.0, n_cell=None,
+               **params):
+        """Queue a request; returns its id.  rain: table in mm per dtrain."""
+        z = np.ascontiguousarray(z, dtype=np.float64)
+        h0 = np.ascontiguousarray(h0, dtype=np.float64)
+        g = Grid(z.shape[0], z.shape[1], float(dx), float(dy if dy is not None else dx),
+                 float(nodata), _dp(z), _dp(h0), None)
+        if n_cell is not None:
+            n_cell = np.ascontiguousarray(n_cell, dtype=np.float64)
+            g.n_cell = _dp(n_cell)
+        params.setdefault("precision", self.precision)
+        p = default_params(**params)
+        r = None if rain is None else np.ascontiguousarray(rain, dtype=np.float64)
+        out = C.c_int()
+        check(lib().sw2d_server_submit(self.h_, C.byref(g), C.byref(p),
+                                       None if r is None else _dp(r), 0 if r is None else len(r),
+                                       float(dtrain), C.byref(out)))
+        self.shapes.append(z.shape)
+        return out.value
+
+    def advance(self, n_iter):
+        """Admit, step n_iter iterations, retire; returns (running, queued)."""
+        nr, nq = C.c_int(), C.c_int()
+        check(lib().sw2d_server_advance(self.h_, int(n_iter), C.byref(nr), C.byref(nq)))
+        return nr.value, nq.value
+
+    def info(self, rid):
+        i = RequestInfo()
+        check(lib().sw2d_server_info(self.h_, int(rid), C.byref(i)))
+        return {k: getattr(i, k) for k, _ in RequestInfo._fields_}
+
+    def read(self, rid, field):
+        r, c = self.shapes[rid]
+        shape = {F_H: (r, c), F_U: (r + 1, c), F_V: (r, c + 1)}[field]
+        out = np.empty(shape, dtype=np.float64)
+        check(lib().sw2d_server_read(self.h_, int(rid), field, _dp(out)))
+        return out
+
+    def stats(self, rid):
+        st = Stats()
+        check(lib().sw2d_server_stats(self.h_, int(rid), C.byref(st)))
+        return {k: getattr(st, k) for k, _ in Stats._fields_}
+
+    def release(self, rid):
+        check(lib().sw2d_server_release(self.h_, int(rid)))

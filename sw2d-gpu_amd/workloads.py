@@ -71,6 +71,81 @@ def config3(size=8192, precision=sw2d.FP32, reservoir=512):
                      "lognormal(0,0.5) per 64^2 block, CFL 0.5" % (size, reservoir))
 
 
+def config4(n_grids=64, per_grid=4, precision=sw2d.FP32):
+    """Batched ensemble: n_grids grids, side = round(exp(U(ln 256, ln 2048)))
+    at dx = 5 m (grid g: seed 2000 + g), fractal relief 0-50 m + 0.5% tilt
+    falling to an open south edge (recorded choice: the configuration fixes
+    only the generator and dx), n ~ U(0.03, 0.10) per cell, dry start.  Each
+    grid carries per_grid hyetographs of 24 five-minute intensities ~
+    Gamma(k = 0.8, theta = 20 mm/h) over 2 h; CFL 0.5 (step <= 2 s).  The
+    scenarios of one grid share its bed and raster on the device."""
+    grids = []
+    for g in range(n_grids):
+        seed = 2000 + g
+        u = float(sw2d.gen_uniform((1,), seed + 101, np.log(256.0), np.log(2048.0))[0])
+        side = int(round(np.exp(u)))
+        z, n = catchment(side, seed, 5.0, 50.0, tilt=0.005)
+        hyet = [sw2d.gen_gamma((24,), seed * 16 + k, 0.8, 20.0) * (300.0 / 3600.0)
+                for k in range(per_grid)]  # mm per 300 s
+        grids.append(dict(z=z, n_cell=n, side=side, hyetographs=hyet))
+    params = dict(dt=1.0, manning=0.05, cfl=0.5, edge_open=sw2d.EDGE_SOUTH,
+                  forcing_mode=sw2d.FORCING_STEP, precision=precision, t_end=7200.0)
+    return dict(grids=grids, dx=5.0, dtrain=300.0, params=params,
+                desc="%d grids x %d hyetographs, sides log-uniform [256, 2048], dx=5, fractal "
+                     "0-50 m + 0.5%% tilt to an open south edge, gamma(0.8, 20 mm/h) 5-min "
+                     "intensities, 2 h, CFL 0.5" % (n_grids, per_grid))
+
+
+def make_batch(w, precision=sw2d.FP32):
+    """All scenarios of a config4 workload in one sw2d.Batch; returns
+    (batch, scenario ids, cells per scenario)."""
+    b = sw2d.Batch(precision)
+    ids, cells = [], []
+    p = dict(w["params"])
+    p["precision"] = precision
+    for g in w["grids"]:
+        first = -1
+        h0 = np.zeros_like(g["z"])
+        for hy in g["hyetographs"]:
+            sid = b.add(g["z"], h0, w["dx"], share_with=first, n_cell=g["n_cell"], **p)
+            b.scenario(sid).set_rain(hy, w["dtrain"])
+            if first < 0:
+                first = sid
+            ids.append(sid)
+            cells.append(g["z"].size)
+    b.touch()
+    return b, ids, np.array(cells, dtype=np.int64)
+
+
+def config5(n_requests=48, rate=50.0, precision=sw2d.FP32):
+    """Serving stream: Poisson arrivals at `rate` per second (arrival times
+    from seed 3000), request r (seed 3000 + r): side log-uniform in
+    [128, 4096] at dx = 5 m, fractal relief 0-50 m + 0.5% tilt to an open
+    south edge (recorded choice), n ~ U(0.02, 0.12) per cell, dry start,
+    duration ~ U(600, 21600) s, gamma(0.8, 20 mm/h) five-minute intensities
+    over the duration; CFL 0.5 (step <= 2 s)."""
+    gaps = sw2d.gen_uniform((n_requests,), 3000, 0.0, 1.0)
+    arrivals = np.cumsum(-np.log1p(-gaps) / rate)
+    reqs = []
+    for r in range(n_requests):
+        seed = 3000 + r
+        u = sw2d.gen_uniform((2,), seed + 101, 0.0, 1.0)
+        side = int(round(np.exp(np.log(128.0) + u[0] * (np.log(4096.0) - np.log(128.0)))))
+        duration = 600.0 + u[1] * (21600.0 - 600.0)
+        z, n = catchment(side, seed, 5.0, 50.0, tilt=0.005, n_range=(0.02, 0.12))
+        n_rain = int(np.ceil(duration / 300.0)) + 1
+        rain = sw2d.gen_gamma((n_rain,), seed * 16, 0.8, 20.0) * (300.0 / 3600.0)
+        reqs.append(dict(z=z, n_cell=n, side=side, duration=duration, rain=rain,
+                         arrival=float(arrivals[r])))
+    params = dict(dt=1.0, manning=0.05, cfl=0.5, edge_open=sw2d.EDGE_SOUTH,
+                  forcing_mode=sw2d.FORCING_STEP, precision=precision)
+    return dict(requests=reqs, dx=5.0, dtrain=300.0, params=params, rate=rate,
+                desc="%d Poisson arrivals at %g/s, sides log-uniform [128, 4096], dx=5, "
+                     "durations U(600, 21600) s, fractal 0-50 m + 0.5%% tilt to an open south "
+                     "edge, n ~ U(0.02, 0.12), gamma(0.8, 20 mm/h) 5-min rain, CFL 0.5"
+                     % (n_requests, rate))
+
+
 def make_sim(w, **over):
     p = dict(w["params"])
     p.update(over)
