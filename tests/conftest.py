@@ -11,6 +11,35 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 sys.path.insert(0, os.path.join(ROOT, "sw2d-gpu_amd"))
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 DAM_BREAK = os.path.join(GOLDEN, "dam_break")
+PERI_LAKE = os.path.join(GOLDEN, "peri_lake_model")
+
+
+def materialize_case(src, db_root, name, setup_edits=None):
+    """Copies a golden case into db_root/<name>/input (gunzipping *.gz),
+    creates its empty output/ directory, applies {key: value} edits to
+    setup.dat and returns the case directory."""
+    import gzip
+    import shutil
+    case = os.path.join(db_root, name)
+    inp = os.path.join(case, "input")
+    os.makedirs(inp, exist_ok=True)
+    os.makedirs(os.path.join(case, "output"), exist_ok=True)
+    for f in os.listdir(os.path.join(src, "input")):
+        a = os.path.join(src, "input", f)
+        if f.endswith(".gz"):
+            with gzip.open(a, "rb") as g, open(os.path.join(inp, f[:-3]), "wb") as o:
+                shutil.copyfileobj(g, o)
+        else:
+            shutil.copyfile(a, os.path.join(inp, f))
+    if setup_edits:
+        path = os.path.join(inp, "setup.dat")
+        lines = open(path, "rb").read().decode().split("\n")
+        for k, v in setup_edits.items():
+            for i, ln in enumerate(lines):
+                if ln.startswith(k + ":"):
+                    lines[i] = "%s: %s%s" % (k, v, "\r" if ln.endswith("\r") else "")
+        open(path, "wb").write("\n".join(lines).encode())
+    return case
 
 
 def pytest_configure(config):
