@@ -22,7 +22,7 @@ struct sw2d_batch {
   hipStream_t stream = nullptr;
   std::vector<sw2d_ctx *> scen;   // index = scenario id; nullptr once removed
   std::vector<int> share_of;      // id whose statics are shared, or -1
-  std::vector<int> rows_override; // manual tile height, 0 = bucket choice
+  std::vector<int> rows_override; // manual segment rows, 0 = bucket choice
   std::vector<char> parked;       // seen at its end time by status: left out of the work-list
   std::map<int, int> bucket_rows; // autotuner result per size bucket
   // device

@@ -102,7 +102,7 @@ struct sw2d_ctx {
   int prec = 0;
   size_t esz = 8;
   int R = 0, C = 0, P = 0, TH = 0, tiles_x = 0, tiles_y = 0, n_tiles = 0;
-  int n_acc = 0;  // accumulator slots: enough for the smallest tile height (4 rows)
+  int n_acc = 0;  // accumulator slots: enough for the shortest segment (4 rows)
   sw2d_params prm{};
   double nodata = 0;
   double dx = 0, dy = 0;
