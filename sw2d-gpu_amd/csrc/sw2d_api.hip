@@ -30,11 +30,11 @@ void sw2d_params_default(sw2d_params *p) {
 
 size_t sw2d_footprint_bytes(int rows, int cols, int precision, int has_n) {
   const size_t esz = precision == SW2D_FP64 ? 8 : 4;
-  const size_t P = round_up((size_t)cols + 1, 64);
+  const size_t P = sw2d::state_pitch(cols);
   const size_t tiles = ((size_t)(cols + sw2d::STRIP_W - 1) / sw2d::STRIP_W) * (((size_t)rows + 3) / 4);
-  size_t cells = (size_t)rows * P, faces = (size_t)(rows + 1) * P;
-  // h, U, V ping-pong pairs and the bed
-  size_t b = esz * (2 * cells + 2 * faces + 2 * cells + cells);
+  size_t cells = (size_t)rows * P, hv = (size_t)(rows + 1) * P, faces = (size_t)(rows + 2) * P;
+  // h, U, V ping-pong pairs (with their padding rows) and the bed
+  size_t b = esz * (2 * hv + 2 * faces + 2 * hv + cells);
   if (has_n) b += esz * cells;
   b += tiles * 2 * sw2d::ACC_N * sizeof(double);
   b += 4 * (((size_t)rows + 63) / 64) * (((size_t)cols + 63) / 64);
@@ -84,7 +84,7 @@ int sw2d_create(const sw2d_grid *grid, const sw2d_params *params, sw2d_ctx **out
   x->esz = x->prec == SW2D_FP64 ? 8 : 4;
   x->R = R;
   x->C = C;
-  x->P = (int)round_up((size_t)C + 1, 64);
+  x->P = (int)sw2d::state_pitch(C);
   x->tiles_x = (C + sw2d::STRIP_W - 1) / sw2d::STRIP_W;
   x->TH = sw2d::default_seg_rows(R, x->tiles_x);
   x->tiles_y = (R + x->TH - 1) / x->TH;
@@ -101,12 +101,13 @@ int sw2d_create(const sw2d_grid *grid, const sw2d_params *params, sw2d_ctx **out
   if (hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking) != hipSuccess)
     return bad(fail(SW2D_E_DEVICE, "hipStreamCreate failed"));
 
-  const size_t cells = (size_t)R * x->P, faces = (size_t)(R + 1) * x->P;
+  // h and V: R rows + 1 padding row; U: R+1 rows + 1 padding row
+  const size_t cells = (size_t)R * x->P, hv = (size_t)(R + 1) * x->P, faces = (size_t)(R + 2) * x->P;
   int rc = 0;
   for (int b = 0; b < 2 && !rc; b++) {
-    rc = rc ? rc : x->alloc(&x->h[b], cells * x->esz);
+    rc = rc ? rc : x->alloc(&x->h[b], hv * x->esz);
     rc = rc ? rc : x->alloc(&x->U[b], faces * x->esz);
-    rc = rc ? rc : x->alloc(&x->V[b], cells * x->esz);
+    rc = rc ? rc : x->alloc(&x->V[b], hv * x->esz);
   }
   rc = rc ? rc : x->alloc(&x->z, cells * x->esz);
   rc = rc ? rc : x->alloc((void **)&x->acc, (size_t)x->n_acc * 2 * sw2d::ACC_N * sizeof(double));
@@ -167,13 +168,13 @@ int sw2d_create(const sw2d_grid *grid, const sw2d_params *params, sw2d_ctx **out
     rc = x->write_pitched(x->ncell, R, C, nv.data(), false);
     if (rc) return bad(rc);
   }
-  rc = x->write_pitched(x->h[0], R, C, grid->h0, true);
+  rc = x->write_h(x->h[0], grid->h0);
   if (rc) return bad(rc);
   for (int b = 0; b < 2; b++) {
-    if (hipMemset(x->U[b], 0, faces * x->esz) != hipSuccess || hipMemset(x->V[b], 0, cells * x->esz) != hipSuccess)
+    if (hipMemset(x->U[b], 0, faces * x->esz) != hipSuccess || hipMemset(x->V[b], 0, hv * x->esz) != hipSuccess)
       return bad(fail(SW2D_E_DEVICE, "state clear"));
   }
-  if (hipMemcpy(x->h[1], x->h[0], cells * x->esz, hipMemcpyDeviceToDevice) != hipSuccess ||
+  if (hipMemcpy(x->h[1], x->h[0], hv * x->esz, hipMemcpyDeviceToDevice) != hipSuccess ||
       hipMemset(x->acc, 0, (size_t)x->n_acc * 2 * sw2d::ACC_N * sizeof(double)) != hipSuccess)
     return bad(fail(SW2D_E_DEVICE, "state init"));
 
@@ -547,9 +548,9 @@ int sw2d_write_state(sw2d_ctx *x, const double *h, const double *U, const double
   int rc = x->fetch_ctrl(c);
   if (rc) return rc;
   const int p = c.parity;
-  rc = x->write_pitched(x->h[p], x->R, x->C, h, true);
-  rc = rc ? rc : x->write_pitched(x->U[p], x->R + 1, x->C, U, false);
-  rc = rc ? rc : x->write_pitched(x->V[p], x->R, x->C + 1, V, false);
+  rc = x->write_h(x->h[p], h);
+  rc = rc ? rc : x->write_U(x->U[p], U);
+  rc = rc ? rc : x->write_V(x->V[p], V);
   if (rc) return rc;
   if (x->prm.cfl > 0) return x->recompute_cfl();
   return SW2D_OK;
@@ -653,8 +654,10 @@ int sw2d_restore(sw2d_ctx *x, const void *buf, size_t size) {
   int rc = x->fetch_ctrl(c);
   if (rc) return rc;
   const unsigned char *in = (const unsigned char *)buf + sizeof(hd);
-  auto put = [&](void *dst, int rows, int cols) -> int {
-    std::vector<unsigned char> tmp((size_t)rows * x->P * x->esz, 0);
+  // the padding of the state arrays (one extra row, columns beyond the
+  // grid) holds the values outside the grid: -1 for h, 0 for U and V
+  auto put = [&](void *dst, int rows, int cols, double pad) -> int {
+    std::vector<unsigned char> tmp = x->pad_rows(rows + 1, pad);
     for (int i = 0; i < rows; i++) {
       std::memcpy(tmp.data() + (size_t)i * x->P * x->esz, in, (size_t)cols * x->esz);
       in += (size_t)cols * x->esz;
@@ -662,9 +665,9 @@ int sw2d_restore(sw2d_ctx *x, const void *buf, size_t size) {
     HIPCHK(hipMemcpy(dst, tmp.data(), tmp.size(), hipMemcpyHostToDevice));
     return SW2D_OK;
   };
-  rc = put(x->h[0], x->R, x->C);
-  rc = rc ? rc : put(x->U[0], x->R + 1, x->C);
-  rc = rc ? rc : put(x->V[0], x->R, x->C + 1);
+  rc = put(x->h[0], x->R, x->C, -1.0);
+  rc = rc ? rc : put(x->U[0], x->R + 1, x->C, 0.0);
+  rc = rc ? rc : put(x->V[0], x->R, x->C + 1, 0.0);
   if (rc) return rc;
   HIPCHK(hipMemcpy(x->acc, in, nacc * sizeof(double), hipMemcpyHostToDevice));
   c.parity = 0;

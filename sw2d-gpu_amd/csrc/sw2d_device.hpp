@@ -2,13 +2,17 @@
 // and the host runtime of the MI355X SW2D core.
 //
 // Memory layout in HBM (one scenario):
-//   all arrays row-major with one pitch P >= cols+1 (multiple of 64 elements)
-//   h[2]      rows   x P   depth, ping-pong; inactive cells hold -1
-//   U[2]      rows+1 x P   discharge on U faces (row i = face north of cell i)
-//   V[2]      rows   x P   discharge on V faces (column j = face west of j,
+//   all arrays row-major with one pitch P >= cols+2 (multiple of 64 elements)
+//   h[2]      rows+1 x P   depth, ping-pong; inactive cells hold -1
+//   U[2]      rows+2 x P   discharge on U faces (row i = face north of cell i)
+//   V[2]      rows+1 x P   discharge on V faces (column j = face west of j,
 //                          column cols = east boundary face)
 //   z         rows   x P   bed (FP32 form: minus a per-64x64-block datum)
 //   ncell     rows   x P   optional Manning raster (FP32 form: g*n^2)
+// The last row of h, U and V and their columns beyond the grid (from cols,
+// or cols+1 for V) are padding that holds the values outside the grid:
+// h = -1 (no cell), U = V = 0 (closed).  The step kernel reads neighbours
+// outside the grid there instead of masking them.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -24,6 +28,10 @@ constexpr int DATUM_BLOCK = 64;     // FP32 bed datum block (cells)
 constexpr int ACC_N = 5;            // rain, source, evap, outflow, clamp
 constexpr int SMAX_SLOTS = 64;      // CFL maximum: workgroups spread their atomics
 constexpr int EDGE_N = 1, EDGE_S = 2, EDGE_W = 4, EDGE_E = 8;
+
+// pitch of every pitched array: the V east face column cols and at least one
+// padding column beyond it, rounded to 64 elements
+__host__ __device__ inline size_t state_pitch(int cols) { return ((size_t)cols + 2 + 63) / 64 * 64; }
 constexpr int ERR_NONFINITE = 1, ERR_TOO_MANY_SRC = 2;
 
 // Per-scenario control block, resident in device memory.  The control

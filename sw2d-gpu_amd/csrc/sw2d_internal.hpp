@@ -194,8 +194,19 @@ struct sw2d_ctx {
       }
     return SW2D_OK;
   }
-  int write_pitched(void *dst, int rows, int cols, const double *in, bool cells) {
-    std::vector<unsigned char> buf((size_t)rows * P * esz, 0);
+  // host staging buffer of `rows` pitched rows, every element `pad`
+  std::vector<unsigned char> pad_rows(int rows, double pad) const {
+    std::vector<unsigned char> buf((size_t)rows * P * esz);
+    if (prec == SW2D_FP64) std::fill((double *)buf.data(), (double *)buf.data() + (size_t)rows * P, pad);
+    else std::fill((float *)buf.data(), (float *)buf.data() + (size_t)rows * P, (float)pad);
+    return buf;
+  }
+  // write a dense host array into a pitched device array of rows + extra
+  // rows; the padding columns and the extra rows hold `pad`.  `cells` marks
+  // cell state: inactive cells are written as -1.
+  int write_pitched(void *dst, int rows, int cols, const double *in, bool cells, int extra = 0,
+                    double pad = 0.0) {
+    std::vector<unsigned char> buf = pad_rows(rows + extra, pad);
     for (int i = 0; i < rows; i++)
       for (int j = 0; j < cols; j++) {
         double v = in[(size_t)i * cols + j];
@@ -207,5 +218,13 @@ struct sw2d_ctx {
     HIPCHK(hipMemcpy(dst, buf.data(), buf.size(), hipMemcpyHostToDevice));
     return SW2D_OK;
   }
+  // The state arrays carry the values outside the grid in their padding, so
+  // the step kernel reads a neighbour outside the grid without masking:
+  // h has one extra row (R) and its padding columns (C .. P-1) all at -1;
+  // U (rows 0 .. R real) has an extra row R+1 and V (columns 0 .. C real) an
+  // extra row R, their padding at 0.
+  int write_h(void *dst, const double *in) { return write_pitched(dst, R, C, in, true, 1, -1.0); }
+  int write_U(void *dst, const double *in) { return write_pitched(dst, R + 1, C, in, false, 1, 0.0); }
+  int write_V(void *dst, const double *in) { return write_pitched(dst, R, C + 1, in, false, 1, 0.0); }
 };
 
