@@ -23,7 +23,7 @@ namespace sw2d {
 constexpr int STRIP_W = 57;         // output columns of a stencil work item
 constexpr int WAVE = 64;            // threads of a stencil workgroup (one wavefront)
 constexpr int BLOCK = 256;          // threads of the whole-grid helper kernels
-constexpr int MAX_TILE_SRC = 64;    // point sources inside one tile region
+constexpr int MAX_TILE_SRC = 256;   // point sources inside one segment and its halo
 constexpr int DATUM_BLOCK = 64;     // FP32 bed datum block (cells)
 constexpr int ACC_N = 5;            // rain, source, evap, outflow, clamp
 constexpr int SMAX_SLOTS = 64;      // CFL maximum: workgroups spread their atomics

@@ -9,21 +9,35 @@
 
 namespace sw2d {
 
-// default rows per strip segment: 32 (measured on 8192^2 grids: 32 and 64
-// rows are within 2 %, 128 and 256 lose 11 % and 26 % to the last round of
-// workgroups), halved to 16 while the grid would give fewer than MIN_ITEMS
-// items, so small grids still fill the chip; a segment pays 7 lead-in steps,
-// so it never gets shorter than 16 rows.  SW2D_SEG_ROWS in the environment
-// overrides it (a multiple of 4 in [4, 4096]).
-constexpr int MIN_ITEMS = 8192;
-inline int default_seg_rows(int rows, int strips) {
+// Default rows per strip segment.  A segment pays 7 lead-in steps, so long
+// segments save work, but the launch runs in rounds of `slots` concurrent
+// wavefronts (CUs x 4 SIMDs x waves per SIMD) and a partly filled last round
+// idles the chip.  The default takes the longest segments that still give
+// about two full rounds (measured on 8192^2 FP32, 4096 slots: 148 rows, two
+// rounds, 104.7 / 95.0 Gcell/s early / wet, against 100.6 / 91.1 for 32 rows
+// and 98.5 / 92.5 for 296 rows in one round), and never fewer than 16 rows.
+// SW2D_SEG_ROWS in the environment overrides it (a multiple of 4 in
+// [4, 4096]).
+inline int default_seg_rows(int rows, int strips, long long slots) {
   if (const char *e = getenv("SW2D_SEG_ROWS")) {
     const int v = atoi(e);
     if (v >= 4 && v <= 4096 && v % 4 == 0) return v;
   }
-  int s = 32;
-  while (s > 16 && (long long)strips * ((rows + s - 1) / s) < MIN_ITEMS) s /= 2;
-  return s;
+  const long long segs = (2 * slots) / (strips > 0 ? strips : 1);
+  long long s = segs > 0 ? (rows + segs - 1) / segs : rows;
+  s = (s + 3) / 4 * 4;
+  return (int)(s < 16 ? 16 : s > 4096 ? 4096 : s);
+}
+
+// concurrent single-wavefront workgroups of the step kernel on the current
+// device: CUs x 4 SIMDs x waves per SIMD (4 for the FP32 kernel, 2 for FP64,
+// set by their register use)
+inline long long step_slots(bool fp64) {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    cus = 256;
+  return (long long)cus * 4 * (fp64 ? 2 : 4);
 }
 
 struct IterLaunch {
