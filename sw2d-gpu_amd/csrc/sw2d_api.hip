@@ -86,7 +86,8 @@ int sw2d_create(const sw2d_grid *grid, const sw2d_params *params, sw2d_ctx **out
   x->C = C;
   x->P = (int)sw2d::state_pitch(C);
   x->tiles_x = (C + sw2d::STRIP_W - 1) / sw2d::STRIP_W;
-  x->TH = sw2d::default_seg_rows(R, x->tiles_x, sw2d::step_slots(x->prec == SW2D_FP64));
+  x->TH = sw2d::default_seg_rows(R, x->tiles_x, sw2d::step_slots(x->prec == SW2D_FP64),
+                                 x->prec == SW2D_FP64 ? 9 : 2, x->prec == SW2D_FP64 ? 64 : 160);
   x->tiles_y = (R + x->TH - 1) / x->TH;
   x->n_tiles = x->tiles_x * x->tiles_y;
   x->n_acc = x->tiles_x * ((R + 3) / 4);

@@ -13,19 +13,30 @@ namespace sw2d {
 // segments save work, but the launch runs in rounds of `slots` concurrent
 // wavefronts (CUs x 4 SIMDs x waves per SIMD) and a partly filled last round
 // idles the chip.  The default takes the longest segments that still give
-// about two full rounds (measured on 8192^2 FP32, 4096 slots: 148 rows, two
-// rounds, 104.7 / 95.0 Gcell/s early / wet, against 100.6 / 91.1 for 32 rows
-// and 98.5 / 92.5 for 296 rows in one round), and never fewer than 16 rows.
+// about `rounds` full rounds, never fewer than 16 rows:
+//  * FP32 (4,096 slots), two rounds: on 8192^2, 148 rows gave 104.7 / 95.0
+//    Gcell/s early / wet, against 100.6 / 91.1 for 32 rows (9 rounds),
+//    102.0 / 94.4 for 64 rows and 98.5 / 92.5 for 296 rows (one round);
+//  * FP64 (2,048 slots), nine rounds: 64 rows gave 42.6 Gcell/s, 32 rows
+//    41.8, 148 rows 38.3 and 296 rows (two rounds) 34.7; the FP64 rows are
+//    twice as wide and long segments lose their halo reuse in L2.
+// On larger grids the number of rounds grows until the segments are no
+// longer than `max_rows` (160 FP32, 64 FP64), the longest measured to pay.
 // SW2D_SEG_ROWS in the environment overrides it (a multiple of 4 in
 // [4, 4096]).
-inline int default_seg_rows(int rows, int strips, long long slots) {
+inline int default_seg_rows(int rows, int strips, long long slots, int rounds, int max_rows) {
   if (const char *e = getenv("SW2D_SEG_ROWS")) {
     const int v = atoi(e);
     if (v >= 4 && v <= 4096 && v % 4 == 0) return v;
   }
-  const long long segs = (2 * slots) / (strips > 0 ? strips : 1);
-  long long s = segs > 0 ? (rows + segs - 1) / segs : rows;
-  s = (s + 3) / 4 * 4;
+  const long long st = strips > 0 ? strips : 1;
+  long long s = 16;
+  for (long long k = rounds; k <= 4096; k++) {
+    const long long segs = k * slots / st;
+    s = segs > 0 ? (rows + segs - 1) / segs : rows;
+    s = (s + 3) / 4 * 4;
+    if (s <= max_rows) break;
+  }
   return (int)(s < 16 ? 16 : s > 4096 ? 4096 : s);
 }
 

@@ -259,7 +259,7 @@ class Sim:
         check(lib().sw2d_profile_begin(self.h_, int(max_iters)))
 
     def profile_end(self):
-        """Summed ms of [control, momentum, continuity] and the iteration count."""
+        """Summed ms of [control kernel, step kernel, 0] and the profiled iteration count."""
         ms = np.zeros(3)
         n = C.c_int64()
         check(lib().sw2d_profile_end(self.h_, _dp(ms), C.byref(n)))
