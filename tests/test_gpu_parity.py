@@ -200,6 +200,35 @@ def test_gpu_snapshot_restore_bitwise():
         assert ta["t"] == tb["t"] and ta["iterations"] == tb["iterations"]
 
 
+def test_gpu_write_state_continues_bitwise():
+    """sw2d_write_state into either ping-pong buffer of another context (the
+    padding around the grid must hold the outside values again) continues
+    the run bit for bit; no forcing, fixed step, so the clock plays no part."""
+    say("case: write_state into a fresh and into a stepped context, FP64 + FP32")
+    for prec in (sw2d.FP64, sw2d.FP32):
+        z, h = random_case(52, 45, 66)
+        mk = lambda: sw2d.Sim(z, h, 2.0, dt=0.05, manning=0.03, precision=prec,
+                              edge_open=sw2d.EDGE_SOUTH | sw2d.EDGE_EAST)
+        a = mk()
+        a.step(30)
+        st = [a.read(f) for f in (sw2d.F_H, sw2d.F_U, sw2d.F_V)]
+        for pre in (0, 1):  # the receiving context's committed buffer: 0 or 1
+            b = mk()
+            if pre:
+                b.step(pre)
+            b.write_state(*st)
+            b.step(40)
+            c = mk()
+            c.write_state(*st)
+            c.step(40)
+            ref = [c.read(f) for f in (sw2d.F_H, sw2d.F_U, sw2d.F_V)]
+            for x, y in zip(ref, [b.read(f) for f in (sw2d.F_H, sw2d.F_U, sw2d.F_V)]):
+                assert np.array_equal(x, y), (prec, pre)
+        a.step(40)
+        for x, y in zip(ref, [a.read(f) for f in (sw2d.F_H, sw2d.F_U, sw2d.F_V)]):
+            assert np.array_equal(x, y), prec
+
+
 def _replay_pair(z, h0, dx, n_iter, *, dt, manning, n_cell, edge_open, rain=None, perturb_bed=None):
     """FP64 run with the CFL step, then FP32 (and optionally FP64 on a
     perturbed bed) replaying its step sequence; yields every 20 iterations."""
