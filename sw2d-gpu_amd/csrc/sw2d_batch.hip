@@ -341,8 +341,22 @@ int sw2d_batch_autotune(sw2d_batch *b, int iters) {
   std::vector<size_t> acc_off(b->n_live + 1, 0);
   for (int g = 0; g < b->n_live; g++)
     acc_off[g + 1] = acc_off[g] + (size_t)b->scen[b->live_ids[g]]->n_acc * 2 * sw2d::ACC_N;
-  double *d_acc_save = nullptr;
-  HIPCHK(hipMalloc((void **)&d_acc_save, std::max<size_t>(1, acc_off.back()) * sizeof(double)));
+  // scratch of the tuning run, released on every return path
+  struct Scratch {
+    double *acc = nullptr;
+    Tile *tw = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~Scratch() {
+      if (acc) (void)hipFree(acc);
+      if (tw) (void)hipFree(tw);
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+    }
+  } scr;
+  HIPCHK(hipEventCreate(&scr.e0));
+  HIPCHK(hipEventCreate(&scr.e1));
+  HIPCHK(hipMalloc((void **)&scr.acc, std::max<size_t>(1, acc_off.back()) * sizeof(double)));
+  double *d_acc_save = scr.acc;
   for (int g = 0; g < b->n_live; g++) {
     sw2d_ctx *c = b->scen[b->live_ids[g]];
     HIPCHK(hipMemcpy(&ctrl_save[g], c->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
@@ -366,10 +380,8 @@ int sw2d_batch_autotune(sw2d_batch *b, int iters) {
   std::sort(order.begin(), order.end());
 
   const std::vector<int> cands{16, 32, 64, 128};
-  hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
-  Tile *d_tw = nullptr;
+  hipEvent_t e0 = scr.e0, e1 = scr.e1;
+  Tile *&d_tw = scr.tw;
   size_t cap_tw = 0;
   int rc = SW2D_OK;
   auto time_choice = [&](float &ms) -> int {
@@ -421,10 +433,6 @@ int sw2d_batch_autotune(sw2d_batch *b, int iters) {
                     hipMemcpyDeviceToDevice);
   }
   (void)hipStreamSynchronize(b->stream);
-  (void)hipFree(d_acc_save);
-  if (d_tw) (void)hipFree(d_tw);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   if (!rc)
     for (auto &kv : choice) b->bucket_rows[kv.first] = kv.second;
   b->dirty = true;
