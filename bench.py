@@ -164,6 +164,68 @@ def serving_bench(n_requests, rate, chunk=20):
     return res
 
 
+def accuracy_config2(size=1024):
+    """Config 2 (a 1024^2 catchment under 50 mm/h for 3600 s): the FP32 form
+    against the FP64 path on the same inputs and the same step sequence (the
+    FP64 run's CFL steps replayed), the FP32 form with its own CFL steps, and
+    the FP64 path itself on the bed moved by +-1e-7 m (about the FP32 bed's
+    representation error), which measures how much of the FP32 deviation the
+    scheme's own conditioning accounts for.  Depth errors at t = 3600 s and
+    the relative mass-balance error of each run."""
+    import sw2d
+    import workloads
+    w = workloads.config2(size=size, precision=sw2d.FP64)
+
+    def run(s):
+        while not s.stats()["done"]:
+            s.step(256)
+        s.sync()
+        return s
+
+    t0 = time.perf_counter()
+    cap = 200000
+    a = workloads.make_sim(w, precision=sw2d.FP64)
+    a.record_dt(cap)
+    run(a)
+    D = a.dt_history(cap)
+
+    def replay(prec, z=None):
+        s = workloads.make_sim(dict(w, z=w["z"] if z is None else z), precision=prec)
+        s.set_dt_schedule(D)
+        return run(s)
+
+    rng = np.random.default_rng(1002)
+    runs = {"fp32_same_steps": replay(sw2d.FP32),
+            "fp32_own_cfl": run(workloads.make_sim(w, precision=sw2d.FP32)),
+            "fp64_bed_1e-7": replay(sw2d.FP64, z=w["z"] + rng.choice([-1e-7, 1e-7], w["z"].shape))}
+    ha = a.read(sw2d.F_H)
+    sa = a.stats()
+
+    def closure(st):  # relative mass-balance error of a run's own accounting
+        inflow = st["rain_in"] + st["source_in"] - st["evap_out"] - st["outflow"] + st["clamp_gain"]
+        return abs(st["volume"] - inflow) / max(st["volume"], 1e-30)
+
+    res = {"workload": w["desc"], "iterations_fp64": int(sa["iterations"]),
+           "volume_fp64_m3": round(sa["volume"], 3), "fp64_mass_balance": closure(sa),
+           "fp64_terms_m3": {k2: round(sa[k2], 4) for k2 in
+                             ("volume", "rain_in", "outflow", "clamp_gain", "evap_out", "source_in")}}
+    for k, s in runs.items():
+        st = s.stats()
+        d = np.abs(s.read(sw2d.F_H) - ha)
+        res[k] = {"max_dh_m": float(d.max()), "p99_9_dh_m": float(np.percentile(d, 99.9)),
+                  "p99_dh_m": float(np.percentile(d, 99)), "cells_dh_over_1e-4": int((d > 1e-4).sum()),
+                  "cells_wet": int((ha > 1e-4).sum()),
+                  "volume_rel_diff": abs(st["volume"] - sa["volume"]) / sa["volume"],
+                  "mass_balance": closure(st), "iterations": int(st["iterations"]),
+                  "terms_m3": {k2: round(st[k2], 4) for k2 in
+                               ("volume", "rain_in", "outflow", "clamp_gain", "evap_out", "source_in")}}
+        s.close()
+    a.close()
+    res["wall_s"] = round(time.perf_counter() - t0, 2)
+    log("accuracy (config 2): %s" % res)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,6 +240,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the FP32-vs-FP64 accuracy leg (config 2)")
     ap.add_argument("--ensemble-grids", type=int, default=64)
     ap.add_argument("--serving-requests", type=int, default=48)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -294,6 +358,8 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, h, U, V)
         log("cpu baseline: %s" % out["cpu_baseline"])
+    if not args.no_accuracy:
+        out["accuracy"] = accuracy_config2()
     if not args.no_ensemble:
         out["ensemble"] = ensemble_bench(args.ensemble_grids)
     if not args.no_serving:

@@ -201,32 +201,35 @@ def test_gpu_snapshot_restore_bitwise():
 
 
 def test_gpu_write_state_continues_bitwise():
-    """sw2d_write_state into either ping-pong buffer of another context (the
-    padding around the grid must hold the outside values again) continues
-    the run bit for bit; no forcing, fixed step, so the clock plays no part."""
-    say("case: write_state into a fresh and into a stepped context, FP64 + FP32")
+    """sw2d_write_state into the second ping-pong buffer of a context, and the
+    same state restored into the first buffer of another (the padding around
+    the grid must hold the outside values in both), continue bit for bit.
+    No forcing and a fixed step: the FP64 path then does not depend on the
+    clock, and the written state also continues the run it was read from."""
+    say("case: write_state into either ping-pong buffer, FP64 + FP32")
     for prec in (sw2d.FP64, sw2d.FP32):
         z, h = random_case(52, 45, 66)
         mk = lambda: sw2d.Sim(z, h, 2.0, dt=0.05, manning=0.03, precision=prec,
                               edge_open=sw2d.EDGE_SOUTH | sw2d.EDGE_EAST)
         a = mk()
         a.step(30)
-        st = [a.read(f) for f in (sw2d.F_H, sw2d.F_U, sw2d.F_V)]
-        for pre in (0, 1):  # the receiving context's committed buffer: 0 or 1
-            b = mk()
-            if pre:
-                b.step(pre)
-            b.write_state(*st)
-            b.step(40)
-            c = mk()
-            c.write_state(*st)
-            c.step(40)
-            ref = [c.read(f) for f in (sw2d.F_H, sw2d.F_U, sw2d.F_V)]
-            for x, y in zip(ref, [b.read(f) for f in (sw2d.F_H, sw2d.F_U, sw2d.F_V)]):
-                assert np.array_equal(x, y), (prec, pre)
-        a.step(40)
-        for x, y in zip(ref, [a.read(f) for f in (sw2d.F_H, sw2d.F_U, sw2d.F_V)]):
+        fields = (sw2d.F_H, sw2d.F_U, sw2d.F_V)
+        st = [a.read(f) for f in fields]
+        b = mk()
+        b.step(1)                 # committed buffer: 1
+        b.write_state(*st)
+        c = mk()
+        c.restore(b.snapshot())   # same state and iteration count, buffer 0
+        b.step(40)
+        c.step(40)
+        for x, y in zip([b.read(f) for f in fields], [c.read(f) for f in fields]):
             assert np.array_equal(x, y), prec
+        if prec == sw2d.FP64:
+            # the FP32 form's rounding of the new depth depends on the
+            # iteration count (its dither is a function of cell and iteration)
+            a.step(40)
+            for x, y in zip([a.read(f) for f in fields], [b.read(f) for f in fields]):
+                assert np.array_equal(x, y)
 
 
 def _replay_pair(z, h0, dx, n_iter, *, dt, manning, n_cell, edge_open, rain=None, perturb_bed=None):
@@ -256,11 +259,13 @@ def _replay_pair(z, h0, dx, n_iter, *, dt, manning, n_cell, edge_open, rain=None
 def test_gpu_fp32_smooth_deep_water():
     """FP32 state on a flow with no wet/dry switching (3-4 m of water over a
     sloping, undulating bed at 40-41 m, a surface hump released at t = 0,
-    Manning raster): depth within 2e-5 m and discharge within 1e-4 of its
+    Manning raster): depth within 2e-5 m and discharge within 2e-4 of its
     maximum of the FP64 run over the same step sequence, volume within 1e-6.
     The discharge bound is the FP32 floor: a 3.5 m depth is stored to
     ~1.2e-7 m, an error that enters the surface gradient g*h*(D/dx)*d(eta)
-    every step (~1.6e-6 m^2/s here, ~3e-5 after 300 random-walk steps)."""
+    every step (~1.6e-6 m^2/s here, ~3e-5 after 300 random-walk steps with
+    round-to-nearest; the FP32 form rounds the new depth without bias, which
+    doubles the variance of that error: 6.5e-5 measured)."""
     say("case: FP32 vs FP64, deep water 160x200, 300 iterations")
     rng = np.random.default_rng(62)
     R, Cc = 160, 200
@@ -278,8 +283,34 @@ def test_gpu_fp32_smooth_deep_water():
     umax = float(max(np.abs(a.read(sw2d.F_U)).max(), np.abs(a.read(sw2d.F_V)).max()))
     say("max|dh| %.3e  max|dU| %.3e  max|dV| %.3e  rel volume %.3e  max|q| %.3e" % (eh, eu, ev, rel, umax))
     assert eh <= 2e-5
-    assert max(eu, ev) <= 1e-4 * umax
+    assert max(eu, ev) <= 2e-4 * umax
     assert rel <= 1e-6
+
+
+def test_gpu_fp32_mass_balance_under_uniform_rain():
+    """Uniform rain on a flat bed, no flow: every cell receives the same
+    increment each step, the worst case for rounding the new depth to
+    nearest (the same rounding error repeats in every cell and every step;
+    2.4e-5 of the volume after 2000 steps on 1024^2).  The FP32 form rounds
+    without bias, so its volume must match its own rain accounting (and the
+    FP64 run) to 1e-7."""
+    say("case: FP32 mass balance, uniform rain on a flat bed 256x256, 2000 iterations")
+    z = np.zeros((256, 256))
+    table = np.full(13, 50.0 * 300 / 3600.0)
+    runs = {}
+    for prec in (sw2d.FP64, sw2d.FP32):
+        s = sw2d.Sim(z, z, 5.0, precision=prec, dt=1.0, manning=0.05, cfl=0.5,
+                     forcing_mode=sw2d.FORCING_STEP)
+        s.set_rain(table, 300.0)
+        s.step(2000)
+        s.sync()
+        runs[prec] = s.stats()
+    a, b = runs[sw2d.FP64], runs[sw2d.FP32]
+    closure = abs(b["volume"] - b["rain_in"]) / b["volume"]
+    rel = abs(a["volume"] - b["volume"]) / a["volume"]
+    say("fp32 closure %.3e  fp32 vs fp64 volume %.3e" % (closure, rel))
+    assert closure <= 1e-7
+    assert rel <= 1e-7
 
 
 def test_gpu_fp32_catchment_within_fp64_conditioning():
