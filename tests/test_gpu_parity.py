@@ -232,6 +232,38 @@ def test_gpu_write_state_continues_bitwise():
                 assert np.array_equal(x, y)
 
 
+@pytest.mark.parametrize("prec", [sw2d.FP64, sw2d.FP32])
+def test_gpu_state_independent_of_segment_rows(prec, monkeypatch):
+    """The step kernel cuts the grid into row segments and recomputes a few
+    lead-in rows above each one.  The state after 60 CFL-limited iterations
+    must be bitwise the same for every segment height (4, 8, 160 rows and the
+    default), which also holds every copy of the step code (lead-in, steady)
+    to the same rounding."""
+    say("case: segment heights 4/8/160/default, precision %d" % prec)
+    fields = (sw2d.F_H, sw2d.F_U, sw2d.F_V)
+    for k, shape in enumerate([(70, 130), (129, 65)]):
+        z, h = random_case(530 + k, *shape, holes=0.02)
+        rng = np.random.default_rng(630 + k)
+        ncell = rng.uniform(0.02, 0.1, shape)
+        rain = rng.gamma(0.8, 20.0, 24)
+        runs = []
+        for rows in (None, 4, 8, 160):
+            if rows is None:
+                monkeypatch.delenv("SW2D_SEG_ROWS", raising=False)
+            else:
+                monkeypatch.setenv("SW2D_SEG_ROWS", str(rows))
+            g = sw2d.Sim(z, h, 2.0, dt=0.5, manning=0.03, n_cell=ncell, cfl=0.45, precision=prec,
+                         edge_open=sw2d.EDGE_SOUTH, INT=0.3, INF=0.1, LWL=0.2, EV_WL_min=0.05)
+            g.set_rain(rain, 300.0)
+            g.step(60)
+            runs.append(([g.read(f) for f in fields], g.stats()["t"]))
+        monkeypatch.delenv("SW2D_SEG_ROWS", raising=False)
+        for st, t in runs[1:]:
+            assert t == runs[0][1], (shape, prec)
+            for x, y in zip(st, runs[0][0]):
+                assert np.array_equal(x, y), (shape, prec, float(np.abs(x - y).max()))
+
+
 def _replay_pair(z, h0, dx, n_iter, *, dt, manning, n_cell, edge_open, rain=None, perturb_bed=None):
     """FP64 run with the CFL step, then FP32 (and optionally FP64 on a
     perturbed bed) replaying its step sequence; yields every 20 iterations."""
