@@ -46,8 +46,10 @@ struct sw2d_batch {
     if (stream) (void)hipStreamDestroy(stream);
   }
 
-  // default segment rows of a scenario: its single-grid choice
-  int default_rows(const sw2d_ctx *c) const { return c->TH; }
+  int batch_rows = 0;              // batch-wide segment height of the live set (rebuild)
+  // default segment rows of a scenario: its single-grid choice, raised to
+  // the batch-wide height of the live set
+  int default_rows(const sw2d_ctx *c) const { return std::max(c->TH, batch_rows); }
   static int bucket_of(const sw2d_ctx *c) {
     const double cells = (double)c->R * c->C;
     return (int)std::floor(std::log2(std::max(cells, 1.0)) / 2.0);
@@ -122,11 +124,16 @@ struct sw2d_batch {
     HIPCHK(hipMemcpy(d_grids, grids.data(), grids.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(d_ctrls, ctrls.data(), ctrls.size() * sizeof(Ctrl *), hipMemcpyHostToDevice));
     std::vector<int> gidx, rows;
+    long long strip_rows = 0;
     for (int g = 0; g < n_live; g++) {
       if (parked[live_ids[g]]) continue;  // finished: its work items would only exit
       gidx.push_back(g);
-      rows.push_back(rows_for(live_ids[g]));
+      const sw2d_ctx *c = scen[live_ids[g]];
+      strip_rows += (long long)c->tiles_x * c->R;
     }
+    const bool fp64 = prec == SW2D_FP64;
+    batch_rows = sw2d::batch_seg_rows(strip_rows, sw2d::step_slots(fp64), fp64 ? 64 : 160);
+    for (int g : gidx) rows.push_back(rows_for(live_ids[g]));
     std::vector<Tile> work;
     build_tiles(gidx, rows, work);
     rc = grow((void **)&d_work, cap_work, std::max<size_t>(1, work.size()) * sizeof(Tile));
@@ -408,16 +415,26 @@ int sw2d_batch_autotune(sw2d_batch *b, int iters) {
     HIPCHK(hipEventElapsedTime(&ms, e0, e1));
     return SW2D_OK;
   };
+  // The incumbent (the batch default or an earlier choice) is timed first
+  // and kept unless a candidate is faster by more than 3 %: one tuning pass
+  // sees a few launches of the current state, and a height that wins there
+  // by less has lost over whole runs (16-32-row choices made on a freshly
+  // wetted ensemble ran 8 % slower than the batch default).
   for (auto &o : order) {
     const int bk = o.second;
-    int best_rows = choice[bk];
-    float best = 0.0f;
+    const int inc = choice[bk];
+    float inc_ms = 0.0f;
+    rc = time_choice(inc_ms);
+    if (rc) break;
+    int best_rows = inc;
+    float best = inc_ms;
     for (int th : cands) {
+      if (th == inc) continue;
       choice[bk] = th;
       float ms = 0.0f;
       rc = time_choice(ms);
       if (rc) break;
-      if (best == 0.0f || ms < best) {
+      if (ms < best && ms < 0.97f * inc_ms) {
         best = ms;
         best_rows = th;
       }

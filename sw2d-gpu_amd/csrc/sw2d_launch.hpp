@@ -40,6 +40,25 @@ inline int default_seg_rows(int rows, int strips, long long slots, int rounds, i
   return (int)(s < 16 ? 16 : s > 4096 ? 4096 : s);
 }
 
+// Segment rows of the scenarios of a batch whose live set holds
+// `strip_rows` strip rows in all (sum over scenarios of strips x rows): the
+// height that gives the whole set about twelve rounds of wave slots, at
+// least 16 and at most `max_rows`.  A batch mixes segments of different
+// cost (wet and dry grids, short last segments), so it needs more rounds
+// than one uniform grid to keep the end of a launch busy.  On the 256
+// scenarios of 64 grids (sides 259-1955, 3.36 M strip rows, FP32), whole
+// 2 h runs took 17.8 s at this height (68 rows), 17.9 s at 52 and 64 rows,
+// 18.4 s at 128, 18.6 s at 32 and 20.4 s at 256 rows; the single-grid
+// heights of the scenarios (16-36 rows) took 19.3 s.  0 when
+// SW2D_SEG_ROWS is set (the override then applies to every scenario).
+inline int batch_seg_rows(long long strip_rows, long long slots, int max_rows) {
+  if (getenv("SW2D_SEG_ROWS")) return 0;
+  const long long per = 12 * (slots > 0 ? slots : 1);
+  long long s = (strip_rows + per - 1) / per;
+  s = (s + 3) / 4 * 4;
+  return (int)(s < 16 ? 16 : s > max_rows ? max_rows : s);
+}
+
 // concurrent single-wavefront workgroups of the step kernel on the current
 // device: CUs x 4 SIMDs x waves per SIMD (4 for the FP32 kernel, 2 for FP64,
 // set by their register use)
