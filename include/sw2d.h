@@ -195,14 +195,19 @@ int sw2d_batch_touch(sw2d_batch *b);
 /* Removes a scenario from the work-list and frees its device memory. */
 int sw2d_batch_remove(sw2d_batch *b, int id);
 /* Segment length (rows) of one scenario's work items: a multiple of 4 in
- * [4, 4096]; 0 = the autotuned choice for its size bucket. */
+ * [4, 4096]; 0 = the autotuned choice for its size bucket, or before any
+ * tuning the batch default: the scenario's single-grid length raised to the
+ * length that gives the whole live set about twelve rounds of the device's
+ * concurrent wavefronts (recomputed when the live set changes). */
 int sw2d_batch_set_tile_rows(sw2d_batch *b, int id, int rows);
 /* Chooses the segment length of each size bucket (floor(log2(cells)/2)):
  * with one iteration of every live scenario decided but not committed, the
- * whole batch's step kernel is timed over `iters` launches for every
- * candidate (16, 32, 64, 128 rows) of one bucket at a time, largest bucket
- * first, and the fastest is kept.  Control blocks and volume accumulators
- * are restored afterwards, so the batch state is bitwise unchanged. */
+ * whole batch's step kernel is timed over `iters` launches for the
+ * bucket's current length (the incumbent) and each candidate (16, 32, 64,
+ * 128 rows), one bucket at a time, largest bucket first.  A candidate
+ * replaces the incumbent only when it is more than 3 % faster.  Control
+ * blocks and volume accumulators are restored afterwards, so the batch
+ * state is bitwise unchanged. */
 int sw2d_batch_autotune(sw2d_batch *b, int iters);
 int sw2d_batch_tile_rows(sw2d_batch *b, int id, int *rows);
 int sw2d_batch_step(sw2d_batch *b, int64_t n_iter);
