@@ -226,6 +226,7 @@ int sw2d_create(const sw2d_grid *grid, const sw2d_params *params, sw2d_ctx **out
     G.ncell = (const T *)x->ncell;
     G.datum = x->datum;
     G.rain_map = nullptr;
+    G.unit = (T)1;
     G.ctrl = x->d_ctrl;
     G.acc = x->acc;
     return hipMemcpy(x->d_grid, &G, sizeof(G), hipMemcpyHostToDevice);

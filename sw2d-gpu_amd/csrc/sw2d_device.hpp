@@ -96,6 +96,7 @@ struct Grid {
   const T *rain_map;              // rain multiplier per block, in the state precision
   Ctrl *ctrl;
   double *acc;                   // per tile: ACC_N x (sum, compensation)
+  T unit;                        // 1: the rain multiplier read where there is no map
 };
 
 // Interpolation shared by rain, point sources and evaporation

@@ -237,8 +237,11 @@ def test_gpu_state_independent_of_segment_rows(prec, monkeypatch):
     """The step kernel cuts the grid into row segments and recomputes a few
     lead-in rows above each one.  The state after 60 CFL-limited iterations
     must be bitwise the same for every segment height (4, 8, 160 rows and the
-    default), which also holds every copy of the step code (lead-in, steady)
-    to the same rounding."""
+    default), which also holds every copy of the step code to the same
+    rounding: lead-in, the general steady step, and (FP32) the body form of
+    rows away from the north and south edges, which segments holding a point
+    source do not use.  The second grid has two point sources, all four edges
+    open and a rain block map."""
     say("case: segment heights 4/8/160/default, precision %d" % prec)
     fields = (sw2d.F_H, sw2d.F_U, sw2d.F_V)
     for k, shape in enumerate([(70, 130), (129, 65)]):
@@ -246,6 +249,10 @@ def test_gpu_state_independent_of_segment_rows(prec, monkeypatch):
         rng = np.random.default_rng(630 + k)
         ncell = rng.uniform(0.02, 0.1, shape)
         rain = rng.gamma(0.8, 20.0, 24)
+        edges = sw2d.EDGE_SOUTH if k == 0 else sw2d.EDGE_NORTH | sw2d.EDGE_SOUTH | sw2d.EDGE_WEST | sw2d.EDGE_EAST
+        bmap = rng.lognormal(0.0, 0.5, ((shape[0] + 7) // 8, (shape[1] + 7) // 8))
+        q = rng.uniform(0.0, 3.0, (12, 2))
+        cells = [40 * shape[1] + 30, 100 * shape[1] + 7]
         runs = []
         for rows in (None, 4, 8, 160):
             if rows is None:
@@ -253,8 +260,12 @@ def test_gpu_state_independent_of_segment_rows(prec, monkeypatch):
             else:
                 monkeypatch.setenv("SW2D_SEG_ROWS", str(rows))
             g = sw2d.Sim(z, h, 2.0, dt=0.5, manning=0.03, n_cell=ncell, cfl=0.45, precision=prec,
-                         edge_open=sw2d.EDGE_SOUTH, INT=0.3, INF=0.1, LWL=0.2, EV_WL_min=0.05)
-            g.set_rain(rain, 300.0)
+                         edge_open=edges, INT=0.3, INF=0.1, LWL=0.2, EV_WL_min=0.05)
+            if k == 0:
+                g.set_rain(rain, 300.0)
+            else:
+                g.set_rain(rain, 300.0, bmap, 8)
+                g.set_sources(q, 60.0, cells)
             g.step(60)
             runs.append(([g.read(f) for f in fields], g.stats()["t"]))
         monkeypatch.delenv("SW2D_SEG_ROWS", raising=False)
