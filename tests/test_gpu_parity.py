@@ -275,6 +275,47 @@ def test_gpu_state_independent_of_segment_rows(prec, monkeypatch):
                 assert np.array_equal(x, y), (shape, prec, float(np.abs(x - y).max()))
 
 
+@pytest.mark.parametrize("shape", [(1, 77), (90, 1), (5, 200), (6, 3), (3, 130), (200, 57), (64, 58)])
+def test_gpu_fp32_odd_shapes(shape, monkeypatch):
+    """The FP32 step on grids with no body rows (fewer than six rows), one
+    strip holding both the west and the east edge (57 columns or fewer),
+    single rows and single columns, with all four edges open, rain, a point
+    source and CFL steps: the state is bitwise the same for segment heights
+    4 and the default, finite, and its volume change matches its own
+    accounting (rain + source - outflow + clamp gain)."""
+    say("case: FP32 odd shape %s" % (shape,))
+    z, h = random_case(800 + shape[0] + shape[1], *shape, holes=0.02)
+    rng = np.random.default_rng(900 + shape[0])
+    edges = sw2d.EDGE_NORTH | sw2d.EDGE_SOUTH | sw2d.EDGE_WEST | sw2d.EDGE_EAST
+    q = rng.uniform(0.0, 2.0, (12, 1))
+    rain = rng.gamma(0.8, 20.0, 24)
+    cell = [int(shape[0] // 2) * shape[1] + int(shape[1] // 2)]
+    fields = (sw2d.F_H, sw2d.F_U, sw2d.F_V)
+    runs = []
+    for rows in (None, 4):
+        if rows is None:
+            monkeypatch.delenv("SW2D_SEG_ROWS", raising=False)
+        else:
+            monkeypatch.setenv("SW2D_SEG_ROWS", str(rows))
+        g = sw2d.Sim(z, h, 2.0, dt=0.2, manning=0.03, cfl=0.45, precision=sw2d.FP32, edge_open=edges,
+                     INT=0.3, INF=0.1, LWL=0.2)
+        g.set_rain(rain, 60.0)
+        g.set_sources(q, 60.0, cell)
+        v0 = g.stats()["volume"]
+        g.step(150)
+        runs.append(([g.read(f) for f in fields], g.stats(), v0))
+    monkeypatch.delenv("SW2D_SEG_ROWS", raising=False)
+    (a, sa, v0), (b, sb, _) = runs
+    assert sa["t"] == sb["t"] and sa["iterations"] == sb["iterations"] == 150
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y), shape
+        assert np.isfinite(x).all(), shape
+    inflow = sa["rain_in"] + sa["source_in"] - sa["evap_out"] - sa["outflow"] + sa["clamp_gain"]
+    err = abs((sa["volume"] - v0) - inflow) / max(sa["volume"], v0)
+    say("volume %.6e -> %.6e, closure %.2e" % (v0, sa["volume"], err))
+    assert err <= 1e-5, (shape, err)
+
+
 def _replay_pair(z, h0, dx, n_iter, *, dt, manning, n_cell, edge_open, rain=None, perturb_bed=None):
     """FP64 run with the CFL step, then FP32 (and optionally FP64 on a
     perturbed bed) replaying its step sequence; yields every 20 iterations."""
