@@ -35,8 +35,6 @@ inline int fail(int code, const std::string &msg) {
     if (e_ != hipSuccess) return fail(SW2D_E_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; }
-
 struct Comp {
   double s = 0, c = 0;
   void add(double x) {
