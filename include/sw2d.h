@@ -172,7 +172,7 @@ double sw2d_forcing_value(double time, const double *table, int n, double dT,
 /* ---- batched ensembles (no reference counterpart: the reference runs the
  * cases of dir.txt one after another, L:775-1526) ---------------------- */
 /* Many scenarios, each a full sw2d_ctx, advanced together: each iteration's
- * step launch covers every live scenario (a work-list of 57-column strip
+ * step launch covers every live scenario (a work-list of 56-column strip
  * segments of `rows` rows, chosen per scenario), and chunks of iterations
  * replay a captured HIP graph.  Every scenario keeps its own clock, step D
  * and end time. */

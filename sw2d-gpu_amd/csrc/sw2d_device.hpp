@@ -20,7 +20,7 @@
 
 namespace sw2d {
 
-constexpr int STRIP_W = 57;         // output columns of a stencil work item
+constexpr int STRIP_W = 56;         // output columns of a stencil work item (224 B: whole 32-byte sectors)
 constexpr int WAVE = 64;            // threads of a stencil workgroup (one wavefront)
 constexpr int BLOCK = 256;          // threads of the whole-grid helper kernels
 constexpr int MAX_TILE_SRC = 256;   // point sources inside one segment and its halo
