@@ -31,7 +31,8 @@ void sw2d_params_default(sw2d_params *p) {
 size_t sw2d_footprint_bytes(int rows, int cols, int precision, int has_n) {
   const size_t esz = precision == SW2D_FP64 ? 8 : 4;
   const size_t P = sw2d::state_pitch(cols);
-  const size_t tiles = ((size_t)(cols + sw2d::STRIP_W - 1) / sw2d::STRIP_W) * (((size_t)rows + 3) / 4);
+  const size_t sw = (size_t)sw2d::strip_width(precision == SW2D_FP64);
+  const size_t tiles = ((size_t)cols + sw - 1) / sw * (((size_t)rows + 3) / 4);
   size_t cells = (size_t)rows * P, hv = (size_t)(rows + 1) * P, faces = (size_t)(rows + 2) * P;
   // h, U, V ping-pong pairs (with their padding rows) and the bed
   size_t b = esz * (2 * hv + 2 * faces + 2 * hv + cells);
@@ -85,7 +86,8 @@ int sw2d_create(const sw2d_grid *grid, const sw2d_params *params, sw2d_ctx **out
   x->R = R;
   x->C = C;
   x->P = (int)sw2d::state_pitch(C);
-  x->tiles_x = (C + sw2d::STRIP_W - 1) / sw2d::STRIP_W;
+  const int sw = sw2d::strip_width(x->prec == SW2D_FP64);
+  x->tiles_x = (C + sw - 1) / sw;
   x->TH = sw2d::default_seg_rows(R, x->tiles_x, sw2d::step_slots(x->prec == SW2D_FP64),
                                  x->prec == SW2D_FP64 ? 9 : 2, x->prec == SW2D_FP64 ? 64 : 160);
   x->tiles_y = (R + x->TH - 1) / x->TH;

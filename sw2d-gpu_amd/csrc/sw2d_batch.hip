@@ -91,7 +91,7 @@ struct sw2d_batch {
       const int th = rows[a];
       int slot = 0;
       for (int i0 = 0; i0 < c->R; i0 += th)
-        for (int j0 = 0; j0 < c->C; j0 += sw2d::STRIP_W, slot++)
+        for (int j0 = 0; j0 < c->C; j0 += sw2d::strip_width(prec == SW2D_FP64), slot++)
           for (size_t m : group) list.push_back(Tile{gidx[m], i0, j0, th, slot, 0, 0, 0});
     }
     const size_t n = list.size(), per = (n + 7) / 8;
