@@ -10,7 +10,69 @@
 
 namespace {
 thread_local std::string g_err;
+
+// Dense <-> pitched conversions on the device (see sw2d::launch_pack).
+template <typename T>
+__global__ void k_pack(T *__restrict__ dst, const double *__restrict__ src, const unsigned char *__restrict__ mask,
+                       int rows, int cols, int P, int total_rows, double pad, int mode,
+                       const float *__restrict__ datum, int datum_cols, double g) {
+  const size_t n = (size_t)total_rows * P;
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(k / (size_t)P), j = (int)(k % (size_t)P);
+    double v = pad;
+    if (i < rows && j < cols) {
+      const size_t s = (size_t)i * cols + j;
+      v = src[s];
+      if (mode == sw2d::PACK_CELLS) {
+        if (!mask[s]) v = -1.0;
+      } else if (mode == sw2d::PACK_BED) {
+        if (!mask[s]) v = 0.0;
+        else if (datum) v -= (double)datum[(size_t)(i / sw2d::DATUM_BLOCK) * datum_cols + j / sw2d::DATUM_BLOCK];
+      } else if (mode == sw2d::PACK_GN2) {
+        v = g * v * v;
+      }
+    }
+    dst[k] = (T)v;
+  }
+}
+
+template <typename T>
+__global__ void k_unpack(double *__restrict__ dst, const T *__restrict__ src, int rows, int cols, int P, int clamp0) {
+  const size_t n = (size_t)rows * cols;
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(k / (size_t)cols), j = (int)(k % (size_t)cols);
+    double v = (double)src[(size_t)i * P + j];
+    if (clamp0 && v < 0.0) v = 0.0;
+    dst[k] = v;
+  }
+}
+
+int conv_blocks(size_t n) { return (int)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 8192)); }
 }  // namespace
+
+namespace sw2d {
+hipError_t launch_pack(int precision, void *dst, const double *src, const unsigned char *mask, int rows, int cols,
+                       int P, int total_rows, double pad, int mode, const float *datum, int datum_cols, double g,
+                       hipStream_t s) {
+  const int nb = conv_blocks((size_t)total_rows * P);
+  if (precision == SW2D_FP64)
+    hipLaunchKernelGGL(k_pack<double>, dim3(nb), dim3(256), 0, s, (double *)dst, src, mask, rows, cols, P,
+                       total_rows, pad, mode, datum, datum_cols, g);
+  else
+    hipLaunchKernelGGL(k_pack<float>, dim3(nb), dim3(256), 0, s, (float *)dst, src, mask, rows, cols, P, total_rows,
+                       pad, mode, datum, datum_cols, g);
+  return hipGetLastError();
+}
+hipError_t launch_unpack(int precision, double *dst, const void *src, int rows, int cols, int P, int clamp0,
+                         hipStream_t s) {
+  const int nb = conv_blocks((size_t)rows * cols);
+  if (precision == SW2D_FP64)
+    hipLaunchKernelGGL(k_unpack<double>, dim3(nb), dim3(256), 0, s, dst, (const double *)src, rows, cols, P, clamp0);
+  else
+    hipLaunchKernelGGL(k_unpack<float>, dim3(nb), dim3(256), 0, s, dst, (const float *)src, rows, cols, P, clamp0);
+  return hipGetLastError();
+}
+}  // namespace sw2d
 
 void sw2d_set_error_text(const std::string &m) { g_err = m; }
 
@@ -124,7 +186,10 @@ int sw2d_create(const sw2d_grid *grid, const sw2d_params *params, sw2d_ctx **out
   // active mask (init_inf L:39-44) and statics
   x->active.resize((size_t)R * C);
   for (size_t k = 0; k < (size_t)R * C; k++) x->active[k] = grid->z[k] != grid->nodata;
-  std::vector<double> zdev((size_t)R * C);
+  rc = x->alloc((void **)&x->d_active, (size_t)R * C);
+  if (rc) return bad(rc);
+  if (hipMemcpy(x->d_active, x->active.data(), (size_t)R * C, hipMemcpyHostToDevice) != hipSuccess)
+    return bad(fail(SW2D_E_DEVICE, "mask upload"));
   x->datum_cols = (C + sw2d::DATUM_BLOCK - 1) / sw2d::DATUM_BLOCK;
   const int datum_rows = (R + sw2d::DATUM_BLOCK - 1) / sw2d::DATUM_BLOCK;
   if (x->prec == SW2D_FP32) {
@@ -144,31 +209,15 @@ int sw2d_create(const sw2d_grid *grid, const sw2d_params *params, sw2d_ctx **out
     if (hipMemcpy(x->datum, x->datum_host.data(), x->datum_host.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
       return bad(fail(SW2D_E_DEVICE, "datum upload"));
   }
-  for (int i = 0; i < R; i++)
-    for (int j = 0; j < C; j++) {
-      size_t k = (size_t)i * C + j;
-      double zv = x->active[k] ? grid->z[k] : 0.0;
-      if (x->prec == SW2D_FP32 && x->active[k]) zv -= x->datum_host[(size_t)(i / 64) * x->datum_cols + j / 64];
-      zdev[k] = zv;
-    }
-  // write_pitched masks inactive cells only for cell state; bed is plain
-  {
-    std::vector<unsigned char> keep;
-    keep.swap(x->active);
-    x->active.assign((size_t)R * C, 1);
-    rc = x->write_pitched(x->z, R, C, zdev.data(), false);
-    x->active.swap(keep);
-    if (rc) return bad(rc);
-  }
+  // the bed: 0 on inactive cells; FP32: relative to its block's datum
+  rc = x->pack(x->z, R, C, grid->z, sw2d::PACK_BED);
+  if (rc) return bad(rc);
   if (grid->n_cell) {
     rc = x->alloc(&x->ncell, cells * x->esz);
     if (rc) return bad(rc);
-    std::vector<double> nv((size_t)R * C);
-    for (size_t k = 0; k < nv.size(); k++) {
-      const double n = grid->n_cell[k];
-      nv[k] = x->prec == SW2D_FP64 ? n : params->g * n * n;  // FP32 form stores g*n^2
-    }
-    rc = x->write_pitched(x->ncell, R, C, nv.data(), false);
+    // the FP32 form stores g n^2
+    rc = x->pack(x->ncell, R, C, grid->n_cell, x->prec == SW2D_FP64 ? sw2d::PACK_PLAIN : sw2d::PACK_GN2, 0, 0.0,
+                 params->g);
     if (rc) return bad(rc);
   }
   rc = x->write_h(x->h[0], grid->h0);
@@ -507,12 +556,8 @@ int sw2d_read(sw2d_ctx *x, int field, double *out) {
   if (rc) return rc;
   const int p = c.parity, R = x->R, C = x->C;
   switch (field) {
-    case SW2D_FIELD_H:
-      rc = x->read_pitched(x->h[p], R, C, out);
-      if (!rc)
-        for (size_t k = 0; k < (size_t)R * C; k++)
-          if (out[k] < 0) out[k] = 0.0;
-      return rc;
+    case SW2D_FIELD_H:  // inactive cells (-1) read as 0
+      return x->read_pitched(x->h[p], R, C, out, true);
     case SW2D_FIELD_U:
       return x->read_pitched(x->U[p], R + 1, C, out);
     case SW2D_FIELD_V:

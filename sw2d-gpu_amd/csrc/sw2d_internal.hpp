@@ -121,6 +121,7 @@ struct sw2d_ctx {
   int vol_blocks = 0;
   int dthist_cap = 0;
   std::vector<unsigned char> active;  // rows*cols
+  unsigned char *d_active = nullptr;  // its device copy
   std::vector<float> datum_host;
   int datum_cols = 0;
   // kernel timing: 4 events per iteration (before control, after control,
@@ -180,41 +181,53 @@ struct sw2d_ctx {
     HIPCHK(hipStreamSynchronize(stream));
     return SW2D_OK;
   }
-  // copy a device array with pitch P into a dense host double array
-  int read_pitched(const void *src, int rows, int cols, double *out) {
-    std::vector<unsigned char> buf((size_t)rows * P * esz);
-    HIPCHK(hipMemcpy(buf.data(), src, buf.size(), hipMemcpyDeviceToHost));
-    for (int i = 0; i < rows; i++)
-      for (int j = 0; j < cols; j++) {
-        size_t k = (size_t)i * P + j;
-        out[(size_t)i * cols + j] = prec == SW2D_FP64 ? ((const double *)buf.data())[k]
-                                                      : (double)((const float *)buf.data())[k];
-      }
+  // Copy a device array with pitch P into a dense host double array: the
+  // conversion runs on the device (clamp0: negative values read as 0), in a
+  // stream-ordered staging buffer, and one copy brings the result back.
+  int read_pitched(const void *src, int rows, int cols, double *out, bool clamp0 = false) {
+    const size_t n = (size_t)rows * cols;
+    if (n == 0) return SW2D_OK;
+    double *d_out = nullptr;
+    HIPCHK(hipMallocAsync((void **)&d_out, n * sizeof(double), stream));
+    hipError_t e = sw2d::launch_unpack(prec, d_out, src, rows, cols, P, clamp0 ? 1 : 0, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, n * sizeof(double), hipMemcpyDeviceToHost, stream);
+    const hipError_t f = hipFreeAsync(d_out, stream);
+    const hipError_t y = hipStreamSynchronize(stream);
+    if (e == hipSuccess) e = f != hipSuccess ? f : y;
+    if (e != hipSuccess) return fail(SW2D_E_DEVICE, std::string("read: ") + hipGetErrorString(e));
     return SW2D_OK;
   }
-  // host staging buffer of `rows` pitched rows, every element `pad`
+  // host staging buffer of `rows` pitched rows, every element `pad` (state
+  // precision; snapshot restore copies raw rows into it)
   std::vector<unsigned char> pad_rows(int rows, double pad) const {
     std::vector<unsigned char> buf((size_t)rows * P * esz);
     if (prec == SW2D_FP64) std::fill((double *)buf.data(), (double *)buf.data() + (size_t)rows * P, pad);
     else std::fill((float *)buf.data(), (float *)buf.data() + (size_t)rows * P, (float)pad);
     return buf;
   }
-  // write a dense host array into a pitched device array of rows + extra
-  // rows; the padding columns and the extra rows hold `pad`.  `cells` marks
-  // cell state: inactive cells are written as -1.
+  // Write a dense host array into a pitched device array of rows + extra
+  // rows; the padding columns and the extra rows hold `pad`.  The array goes
+  // up once, in a stream-ordered staging buffer, and the conversion (mode:
+  // sw2d::PackMode) runs on the device.
+  int pack(void *dst, int rows, int cols, const double *in, int mode, int extra = 0, double pad = 0.0,
+           double g = 0.0) {
+    const size_t n = (size_t)rows * cols;
+    double *d_in = nullptr;
+    HIPCHK(hipMallocAsync((void **)&d_in, std::max<size_t>(n, 1) * sizeof(double), stream));
+    hipError_t e = n ? hipMemcpyAsync(d_in, in, n * sizeof(double), hipMemcpyHostToDevice, stream) : hipSuccess;
+    if (e == hipSuccess)
+      e = sw2d::launch_pack(prec, dst, d_in, d_active, rows, cols, P, rows + extra, pad, mode, datum, datum_cols, g,
+                            stream);
+    const hipError_t f = hipFreeAsync(d_in, stream);
+    const hipError_t y = hipStreamSynchronize(stream);
+    if (e == hipSuccess) e = f != hipSuccess ? f : y;
+    if (e != hipSuccess) return fail(SW2D_E_DEVICE, std::string("write: ") + hipGetErrorString(e));
+    return SW2D_OK;
+  }
+  // `cells` marks cell state: inactive cells are written as -1
   int write_pitched(void *dst, int rows, int cols, const double *in, bool cells, int extra = 0,
                     double pad = 0.0) {
-    std::vector<unsigned char> buf = pad_rows(rows + extra, pad);
-    for (int i = 0; i < rows; i++)
-      for (int j = 0; j < cols; j++) {
-        double v = in[(size_t)i * cols + j];
-        if (cells && !active[(size_t)i * C + j]) v = -1.0;
-        size_t k = (size_t)i * P + j;
-        if (prec == SW2D_FP64) ((double *)buf.data())[k] = v;
-        else ((float *)buf.data())[k] = (float)v;
-      }
-    HIPCHK(hipMemcpy(dst, buf.data(), buf.size(), hipMemcpyHostToDevice));
-    return SW2D_OK;
+    return pack(dst, rows, cols, in, cells ? sw2d::PACK_CELLS : sw2d::PACK_PLAIN, extra, pad);
   }
   // The state arrays carry the values outside the grid in their padding, so
   // the step kernel reads a neighbour outside the grid without masking:
