@@ -168,13 +168,19 @@ int sw2d_create(const sw2d_grid *grid, const sw2d_params *params, sw2d_ctx **out
 
   // h and V: R rows + 1 padding row; U: R+1 rows + 1 padding row
   const size_t cells = (size_t)R * x->P, hv = (size_t)(R + 1) * x->P, faces = (size_t)(R + 2) * x->P;
+  // The step reads every pitched row from column j0 - 4 of its strip (j0 a
+  // multiple of the strip width) and writes from column j0.  The state rows
+  // start on a 256-byte boundary, so the written row pieces are whole 32-byte
+  // sectors in FP32; the bed and raster are only read, and start
+  // STATIC_SHIFT_COLS elements into their block, so their row reads are.
+  constexpr size_t STATE_SHIFT_COLS = 0, STATIC_SHIFT_COLS = 4;
   int rc = 0;
   for (int b = 0; b < 2 && !rc; b++) {
-    rc = rc ? rc : x->alloc(&x->h[b], hv * x->esz);
-    rc = rc ? rc : x->alloc(&x->U[b], faces * x->esz);
-    rc = rc ? rc : x->alloc(&x->V[b], hv * x->esz);
+    rc = rc ? rc : x->alloc(&x->h[b], hv * x->esz, STATE_SHIFT_COLS * x->esz);
+    rc = rc ? rc : x->alloc(&x->U[b], faces * x->esz, STATE_SHIFT_COLS * x->esz);
+    rc = rc ? rc : x->alloc(&x->V[b], hv * x->esz, STATE_SHIFT_COLS * x->esz);
   }
-  rc = rc ? rc : x->alloc(&x->z, cells * x->esz);
+  rc = rc ? rc : x->alloc(&x->z, cells * x->esz, STATIC_SHIFT_COLS * x->esz);
   rc = rc ? rc : x->alloc((void **)&x->acc, (size_t)x->n_acc * 2 * sw2d::ACC_N * sizeof(double));
   x->vol_blocks = x->blocks_for((long long)R * C);
   rc = rc ? rc : x->alloc((void **)&x->partial, (size_t)x->vol_blocks * 2 * sizeof(double));
@@ -213,7 +219,7 @@ int sw2d_create(const sw2d_grid *grid, const sw2d_params *params, sw2d_ctx **out
   rc = x->pack(x->z, R, C, grid->z, sw2d::PACK_BED);
   if (rc) return bad(rc);
   if (grid->n_cell) {
-    rc = x->alloc(&x->ncell, cells * x->esz);
+    rc = x->alloc(&x->ncell, cells * x->esz, STATIC_SHIFT_COLS * x->esz);
     if (rc) return bad(rc);
     // the FP32 form stores g n^2
     rc = x->pack(x->ncell, R, C, grid->n_cell, x->prec == SW2D_FP64 ? sw2d::PACK_PLAIN : sw2d::PACK_GN2, 0, 0.0,

@@ -135,24 +135,39 @@ struct sw2d_ctx {
     if (stream) (void)hipStreamSynchronize(stream);
     graph.reset();
     for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
-    for (void *p : allocs) (void)hipFree(p);
+    for (void *p : allocs) (void)hipFree(base_of(p));
     if (stream) (void)hipStreamDestroy(stream);
   }
 
-  int alloc(void **p, size_t bytes) {
-    hipError_t e = hipMalloc(p, bytes ? bytes : 8);
+  // Pointers handed out `shift` bytes into their allocation (see alloc);
+  // allocs keeps the handed-out pointers.
+  std::vector<std::pair<void *, size_t>> shifted;
+  void *base_of(void *p) const {
+    for (const auto &s : shifted)
+      if (s.first == p) return (char *)p - s.second;
+    return p;
+  }
+  // `shift`: the pointer handed out lies that many bytes into a block as much
+  // larger (the pitched statics use it to move their row reads onto sector
+  // boundaries)
+  int alloc(void **p, size_t bytes, size_t shift = 0) {
+    void *raw = nullptr;
+    hipError_t e = hipMalloc(&raw, (bytes ? bytes : 8) + shift);
     if (e != hipSuccess) {
       *p = nullptr;
       return fail(SW2D_E_MEMORY, std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
     }
+    *p = (char *)raw + shift;
+    if (shift) shifted.emplace_back(*p, shift);
     allocs.push_back(*p);
     return SW2D_OK;
   }
   void release(void *p) {
     if (!p) return;
     auto it = std::find(allocs.begin(), allocs.end(), p);
-    if (it != allocs.end()) allocs.erase(it);
-    (void)hipFree(p);
+    if (it == allocs.end()) return;  // not this context's allocation
+    allocs.erase(it);
+    (void)hipFree(base_of(p));
   }
 
   int fetch_ctrl(Ctrl &c) {
