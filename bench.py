@@ -247,7 +247,34 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy 5 times (the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--stream-ceiling", action="store_true",
+                    help="only time plain streaming kernels (torch's device copy and a two-input "
+                         "add, 1 and 4 GiB per array) to give the HBM rate such kernels reach")
     args = ap.parse_args()
+    if args.stream_ceiling:
+        import torch
+        out = {}
+        for gib in (1, 4):
+            n = gib * (1 << 30) // 4
+            a = torch.rand(n, device="cuda")
+            b = torch.rand(n, device="cuda")
+            c = torch.empty_like(a)
+            for name, fn, nbytes in (("copy", lambda: c.copy_(a), 2 * 4 * n),
+                                     ("add", lambda: torch.add(a, b, out=c), 3 * 4 * n)):
+                for _ in range(3):
+                    fn()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                torch.cuda.synchronize()
+                e0.record()
+                for _ in range(20):
+                    fn()
+                e1.record()
+                torch.cuda.synchronize()
+                out["%s_%dGiB_TBps" % (name, gib)] = round(nbytes * 20 / (e0.elapsed_time(e1) * 1e-3) / 1e12, 3)
+            del a, b, c
+            torch.cuda.empty_cache()
+        print(json.dumps(out), flush=True)
+        return
     if args.copy_calibration:
         import torch
         n = 256 * 1024 * 1024 // 4
