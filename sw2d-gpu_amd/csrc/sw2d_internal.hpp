@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <utility>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -168,6 +169,9 @@ struct sw2d_ctx {
     if (it == allocs.end()) return;  // not this context's allocation
     allocs.erase(it);
     (void)hipFree(base_of(p));
+    shifted.erase(std::remove_if(shifted.begin(), shifted.end(),
+                                 [p](const std::pair<void *, size_t> &s) { return s.first == p; }),
+                  shifted.end());
   }
 
   int fetch_ctrl(Ctrl &c) {
