@@ -151,7 +151,7 @@ int sw2d_create(const sw2d_grid *grid, const sw2d_params *params, sw2d_ctx **out
   const int sw = sw2d::strip_width(x->prec == SW2D_FP64);
   x->tiles_x = (C + sw - 1) / sw;
   x->TH = sw2d::default_seg_rows(R, x->tiles_x, sw2d::step_slots(x->prec == SW2D_FP64),
-                                 x->prec == SW2D_FP64 ? 9 : 2, x->prec == SW2D_FP64 ? 64 : 160);
+                                 x->prec == SW2D_FP64 ? 9 : 3, x->prec == SW2D_FP64 ? 64 : 160);
   x->tiles_y = (R + x->TH - 1) / x->TH;
   x->n_tiles = x->tiles_x * x->tiles_y;
   x->n_acc = x->tiles_x * ((R + 3) / 4);

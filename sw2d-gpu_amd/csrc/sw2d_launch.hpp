@@ -14,9 +14,10 @@ namespace sw2d {
 // wavefronts (CUs x 4 SIMDs x waves per SIMD) and a partly filled last round
 // idles the chip.  The default takes the longest segments that still give
 // about `rounds` full rounds, never fewer than 16 rows:
-//  * FP32 (4,096 slots), two rounds: on 8192^2, 148 rows gave 104.7 / 95.0
-//    Gcell/s early / wet, against 100.6 / 91.1 for 32 rows (9 rounds),
-//    102.0 / 94.4 for 64 rows and 98.5 / 92.5 for 296 rows (one round);
+//  * FP32 (4,096 slots), three rounds: whole 8192^2 runs, interleaved on one
+//    MI355X, took 100 rows (2.94 rounds) to 115.0 Gcell/s, 76 rows (3.9
+//    rounds) to 114.2 and 152 rows (1.94 rounds) to 112.0; 296 rows (one
+//    round) and 32 rows lost 3-6 % and 4 % on an earlier form of the step;
 //  * FP64 (2,048 slots), nine rounds: 64 rows gave 42.6 Gcell/s, 32 rows
 //    41.8, 148 rows 38.3 and 296 rows (two rounds) 34.7; the FP64 rows are
 //    twice as wide and long segments lose their halo reuse in L2.
