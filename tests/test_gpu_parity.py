@@ -1,8 +1,8 @@
 """GPU parity: the HIP path (through the C ABI) against the FP64 oracle.
 
-FP64 gate: max |dh|, |dU|, |dV| <= 1e-10 after the stated iterations
-(differences come only from last-bit pow differences
-between the device and host math libraries).
+FP64 gate: max |dh|, |dU|, |dV| <= 1e-10 after the stated iterations (the
+floating-point tolerance of the FP64 path; differences come only from
+last-bit pow differences between the device and host math libraries).
 """
 import gzip
 import os
