@@ -286,7 +286,7 @@ def main():
         print(json.dumps({"copy_bytes": n * 4, "copies": 5}), flush=True)
         return
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only; multi-GPU sharding is a design "
+        print("bench.py: the library runs on one GPU only; multi-GPU sharding is a design "
               "(row strips with a 3-row halo), not implemented", file=sys.stderr)
         sys.exit(2)
 

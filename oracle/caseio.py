@@ -1,6 +1,6 @@
 """Restatement of the SW2D-GPU case-file readers and result writers.
 
-TEST INFRASTRUCTURE ONLY: the checker for the product's C++ reader/writer
+TEST INFRASTRUCTURE ONLY: what the tests hold the product's C++ reader/writer to
 (sw2d-gpu_amd/csrc/sw2d_io.cpp) and the way tests feed the oracle.
 
 Reference: SW2D-GPU_linux/src/SW2D-GPU.cu
