@@ -30,6 +30,17 @@ EXP = 1.3333330   # L:195, L:312
 _libm_pow = np.vectorize(math.pow, otypes=[np.float64])
 
 
+def _interp(table, t, dT):
+    """The forcing formula shared by stream_flow (L:533-538, L:547-552) and
+    evaporation_load (L:592-597), written out as the reference does: a ramp
+    a[0]*t up to t = 1 s, then a[it] + (a[it+1] - a[it]) / (dT*(t - dT*it))
+    with it = int(t/dT).  Callers keep it + 1 inside the table."""
+    if t <= 1.00:
+        return table[0] * t
+    it = int(t / dT)
+    return table[it] + (table[it + 1] - table[it]) / (dT * (t - dT * it))
+
+
 class RefLayoutSim:
     """State of one case in the reference layout (all float64)."""
 
@@ -267,15 +278,36 @@ class RefLayoutSim:
         self.vno[:N] = self.vn[:N]
         self.ho[:N] = self.h[:N]
 
-    def iterate(self, rain_mm=None, dtrain=60.0):
+    def _forcing(self, rain_mm, sources, evap):
+        """stream_flow L:529-555 and evaporation_load L:589-601 at self.time.
+
+        sources: (qq, dtoq, cells) with qq[n_rows][nst] in m3/s and the flat
+        index of each source cell (L:545); each source assigns its rate to
+        h_ql, in source order, so a later source on the same cell wins.
+        evap: (evapo_mm, dtrain), the evaporation table of
+        gpu_evaporation_calc in mm per dtrain; None leaves Ev untouched
+        (evaporation_on != 1, L:1283-1286)."""
+        t = self.time
+        self.rr = rain_mm(t) if rain_mm else 0.0
+        if sources is not None:
+            qq, dtoq, cells = sources
+            for k, cell in enumerate(cells):
+                ql = _interp([row[k] for row in qq], t, dtoq)
+                self.ql[cell] = ql / (self.dx * self.dy)          # L:540, L:545
+        if evap is not None:
+            table, dtr = evap
+            self.Ev = _interp(table, t, dtr) / (dtr * 1000.0)    # L:592-599
+
+    def iterate(self, rain_mm=None, dtrain=60.0, sources=None, evap=None):
         """One loop iteration L:1268-1336 with uniform rain given as a
-        function of time returning rr in m/s (None = no rain)."""
+        function of time returning rr in m/s (None = no rain), optional
+        point sources and evaporation (see _forcing)."""
         self.flux()
-        self.rr = rain_mm(self.time) if rain_mm else 0.0
+        self._forcing(rain_mm, sources, evap)
         self.continuity()
         self.treat_error()
         self.time = self.time + self.dt
-        self.rr = rain_mm(self.time) if rain_mm else 0.0
+        self._forcing(rain_mm, sources, evap)
         self.continuity()
         self._hm_hn()
         self.uu1_vv1()

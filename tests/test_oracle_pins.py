@@ -19,20 +19,32 @@ import sw2d_np as NP  # noqa: E402
 G, TH = 9.80, 1.0e-4
 
 
-def run_pair(z, h0, dx, dt, n, iters, rain=None, **kw):
+def run_pair(z, h0, dx, dt, n, iters, rain=None, sources=None, evap=None, **kw):
+    """The C oracle and the literal transcription on the same inputs.
+    rain: (table_mm, dtrain); sources: (q[n_rows][nst], dtoq, flat cells);
+    evap: evaporation table in mm per dtrain (needs rain's dtrain, and turns
+    evaporation on)."""
     act = (z != -9999.0).astype(np.uint8)
+    if evap is not None:
+        kw["evaporation_on"] = 1
     o = O.OracleSim(z, act, h0, dx, dt, n, **kw)
     r = NP.RefLayoutSim(z, h0, -9999.0, dx, dt, n, **{k: v for k, v in kw.items()
                                                      if k in ("INT", "INF", "LWL", "EV_WL_min")})
+    rain_fn, np_src, np_ev = None, None, None
     if rain is not None:
         table, dtrain = rain
         o.set_rain(table, dtrain)
+        rain_fn = lambda t: NP._interp(table, t, dtrain) / (dtrain * 1000.0)  # noqa: E731
+    if sources is not None:
+        q, dtoq, cells = sources
+        o.set_sources(np.asarray(q, dtype=np.float64), dtoq, cells)
+        np_src = (q, dtoq, cells)
+    if evap is not None:
+        o.set_evaporation(evap)
+        np_ev = (evap, rain[1])
     for _ in range(iters):
         o.step(1)
-        if rain is not None:
-            r.iterate(lambda t: O.stream_rate(t, table, dtrain, 0) / (dtrain * 1000.0))
-        else:
-            r.iterate()
+        r.iterate(rain_fn, sources=np_src, evap=np_ev)
     return o, r
 
 
@@ -179,12 +191,11 @@ def test_friction_decay_single_step():
 
 
 def test_rain_on_still_water_exact():
-    """Still water + uniform rain: h grows by D*rr*p each iteration, with p
-    switching at EV_WL_min (L:627-634); the accounting matches."""
+    """Still water + uniform rain: h grows by D*rr*p each iteration (L:654)
+    and the accounting matches.  The switch of p at EV_WL_min is pinned by
+    test_evaporation_and_runoff_fraction_on_still_water."""
     z = np.zeros((6, 7))
     h = np.full((6, 7), 0.5)
-    h[0, :] = 0.01  # below EV_WL_min: p = 1 - (INT + INF)
-    h[:] = 0.5
     table = np.full(10, 6.0)   # 6 mm per 60 s
     o = O.OracleSim(z, np.ones(z.shape, np.uint8), h, 1.0, 0.5, 0.03, INT=0.45, INF=0.2,
                     LWL=0.3, EV_WL_min=0.05, forcing_mode=1)
@@ -286,3 +297,208 @@ def test_cfl_step_on_still_water():
     o2 = O.OracleSim(z, np.ones(z.shape, np.uint8), h, 3.0, 0.01, 0.03, cfl=0.5)
     o2.step(1)
     assert o2.stats()["last_D"] == 0.02
+
+
+def test_oracle_matches_literal_with_sources_and_evaporation():
+    """Point sources (two on one cell: the later one wins, L:540-545) and
+    evaporation (L:589-601, switched by EV_WL_min at L:627-634) with rain:
+    the C oracle and the literal transcription agree bitwise."""
+    z, h = random_case(17, 12, 15, holes=0.04)
+    R, Cc = z.shape
+    cells = [3 * Cc + 4, 7 * Cc + 11, 3 * Cc + 4, 10 * Cc + 2]
+    q = [[0.30, -0.10, 0.05, 0.20], [0.10, -0.25, 0.40, 0.15], [0.00, -0.05, 0.60, 0.25],
+         [0.20, -0.15, 0.10, 0.05], [0.35, -0.20, 0.30, 0.10], [0.15, -0.05, 0.20, 0.30]]
+    rain = (np.array([3.0, 5.0, 4.0, 1.0]), 7.0)
+    evap = np.array([1.5, 2.5, 0.5, 2.0])
+    o, r = run_pair(z, h, 2.0, 0.03, 0.05, 300, rain=rain, sources=(q, 5.0, cells), evap=evap,
+                    INT=0.25, INF=0.15, LWL=0.1, EV_WL_min=0.2)
+    assert_same(o, r)
+    # the sources and the evaporation did act
+    o2, r2 = run_pair(z, h, 2.0, 0.03, 0.05, 300, rain=rain, INT=0.25, INF=0.15, LWL=0.1,
+                      EV_WL_min=0.2)
+    assert not np.array_equal(o.read(O.H), o2.read(O.H))
+    st = o.stats()
+    assert st["source_in"] != 0.0 and st["evap_out"] > 0.0
+
+
+def test_point_source_on_still_water_closed_form():
+    """Still water (flat surface, no flow) with point sources: one iteration
+    raises each source cell by exactly D q / (dx dy) (L:540, L:654); two
+    sources on one cell do not add up, the later one wins (L:545 assigns);
+    a cell index outside the grid is ignored; the other cells keep their
+    depth bitwise.  Dyadic values make every expected depth exact."""
+    R, Cc, dx = 5, 6, 2.0
+    z = np.zeros((R, Cc))
+    h = np.full((R, Cc), 0.5)
+    c1, c2 = 1 * Cc + 2, 3 * Cc + 5
+    q = np.array([[0.25, 0.5, 0.125, 3.0]])       # m3/s; the last source is outside
+    cells = [c1, c2, c1, R * Cc]
+    o = O.OracleSim(z, np.ones(z.shape, np.uint8), h, dx, 0.5, 0.03, forcing_mode=1)
+    o.set_sources(q, 10.0, cells)
+    o.step(1)
+    D = 1.0
+    got = o.read(O.H).ravel()
+    expect = np.full(R * Cc, 0.5)
+    expect[c1] = 0.5 + D * 0.125 / (dx * dx)     # 0.53125: the later source wins
+    expect[c2] = 0.5 + D * 0.5 / (dx * dx)       # 0.625
+    assert np.array_equal(got, expect)
+    assert o.stats()["source_in"] == D * (0.125 + 0.5)
+    # reference forcing formula: the ramp q[0] * t for t <= 1 s (L:533-534),
+    # so the first continuity (t = 0) adds nothing and the second (t = dt)
+    # adds D q t / (dx dy)
+    o2 = O.OracleSim(z, np.ones(z.shape, np.uint8), h, dx, 0.25, 0.03, forcing_mode=0)
+    o2.set_sources(np.array([[0.25], [0.25]]), 10.0, [c2])
+    o2.step(1)
+    got2 = o2.read(O.H).ravel()
+    assert got2[c2] == 0.5 + 0.5 * (0.25 * 0.25) / (dx * dx)    # 0.5078125
+    assert np.count_nonzero(got2 != 0.5) == 1
+
+
+def test_evaporation_and_runoff_fraction_on_still_water():
+    """Still water with rain and evaporation (L:627-660): a cell whose depth
+    at the start of the iteration is above EV_WL_min loses D ev and gains
+    D rr (1 - LWL); a cell at or below it gains D rr (1 - (INT + INF)) and
+    does not evaporate.  The free surface is flat (bed = 0.5 - depth) so no
+    face carries flow."""
+    depths = np.array([0.5, 0.03125, 0.0625, 0.25, 0.0625 + 2.0 ** -20])
+    EVmin = 0.0625
+    h = np.repeat(depths[:, None], 4, axis=1)
+    z = 0.5 - h
+    INT, INF, LWL = 0.25, 0.125, 0.375
+    dtrain = 60.0
+    rain_mm, ev_mm = 6.0, 1.5
+    o = O.OracleSim(z, np.ones(z.shape, np.uint8), h, 1.0, 0.5, 0.03, INT=INT, INF=INF, LWL=LWL,
+                    EV_WL_min=EVmin, evaporation_on=1, forcing_mode=1)
+    o.set_rain(np.full(4, rain_mm), dtrain)
+    o.set_evaporation(np.full(4, ev_mm))
+    o.step(1)
+    D = 1.0
+    rr = rain_mm / (dtrain * 1000.0)
+    ev = ev_mm / (dtrain * 1000.0)
+    hi = depths > EVmin
+    assert hi.tolist() == [True, False, False, True, True]
+    expect = np.where(hi, depths - D * ev + D * rr * (1.0 - LWL), depths + D * rr * (1.0 - (INT + INF)))
+    got = o.read(O.H)
+    assert np.allclose(got, expect[:, None], rtol=0, atol=2e-16), np.abs(got - expect[:, None]).max()
+    # a wrong switch (h >= EV_WL_min) or wrong fraction would move a row by
+    # at least D rr * 0.25 = 2.5e-5
+    assert not o.read(O.U).any() and not o.read(O.V).any()
+    st = o.stats()
+    assert abs(st["evap_out"] - D * ev * 4 * hi.sum()) < 1e-18
+    p = np.where(hi, 1.0 - LWL, 1.0 - (INT + INF))
+    assert abs(st["rain_in"] - D * rr * 4 * p.sum()) < 1e-18
+    # evaporation off (L:1283): the table is ignored
+    o2 = O.OracleSim(z, np.ones(z.shape, np.uint8), h, 1.0, 0.5, 0.03, INT=INT, INF=INF, LWL=LWL,
+                     EV_WL_min=EVmin, evaporation_on=0, forcing_mode=1)
+    o2.set_rain(np.full(4, rain_mm), dtrain)
+    o2.set_evaporation(np.full(4, ev_mm))
+    o2.step(1)
+    expect2 = depths + D * rr * p
+    assert np.allclose(o2.read(O.H), expect2[:, None], rtol=0, atol=2e-16)
+    assert o2.stats()["evap_out"] == 0.0
+
+
+@pytest.mark.parametrize("nval", [0.03, 0.0731, 0.11])
+def test_uniform_raster_equals_scalar_manning(nval):
+    """A uniform Manning raster gives the scalar manning_coef run bitwise:
+    the face value sqrt(0.5 (n^2 + n^2)) is n exactly in binary floating
+    point (sqrt of a correctly rounded square returns |n|)."""
+    z, h = random_case(23, 14, 19, holes=0.05)
+    act = (z != -9999.0).astype(np.uint8)
+    a = O.OracleSim(z, act, h, 1.5, 0.02, nval, edge_open=O.EDGE_SOUTH)
+    b = O.OracleSim(z, act, h, 1.5, 0.02, 0.5, edge_open=O.EDGE_SOUTH,
+                    n_cell=np.full(z.shape, nval))
+    a.step(200)
+    b.step(200)
+    for f in (O.H, O.U, O.V):
+        assert np.array_equal(a.read(f), b.read(f))
+
+
+def _friction_face(n_a, n_b, along_rows):
+    """One face with only friction acting: a column (along_rows) or a row of
+    8 cells, flat bed, depth 2, discharge 0.4 on every interior face; the
+    middle face lies between cells 3 and 4.  Returns (U or V at that face
+    after one iteration, the step D, uu1 at the face)."""
+    n = np.full(8, 0.05)
+    n[3], n[4] = n_a, n_b
+    if along_rows:
+        shape, ncell = (8, 1), n[:, None]
+    else:
+        shape, ncell = (1, 8), n[None, :]
+    z = np.zeros(shape)
+    h = np.full(shape, 2.0)
+    o = O.OracleSim(z, np.ones(shape, np.uint8), h, 1.0, 0.01, 0.5, n_cell=ncell)
+    U = np.zeros((shape[0] + 1, shape[1]))
+    V = np.zeros((shape[0], shape[1] + 1))
+    if along_rows:
+        U[1:8, 0] = 0.4
+    else:
+        V[0, 1:8] = 0.4
+    o.set_state(h, U, V)
+    o.step(1)
+    got = o.read(O.U)[4, 0] if along_rows else o.read(O.V)[0, 4]
+    return got, 0.02, 0.4 / 2.0
+
+
+@pytest.mark.parametrize("along_rows", [True, False])
+def test_manning_raster_face_rule(along_rows):
+    """Face roughness of the raster: n_face^2 = (n_a^2 + n_b^2) / 2 in the
+    implicit friction update U1 = U0 (1 - D ram/2) / (1 + D ram/2), ram =
+    g n_face^2 |u| / hf^1.3333330 (L:194-203 for U, L:311-320 for V).  The
+    expected value is formed from the mean of the squares directly; the
+    arithmetic mean of n, or either cell's n alone, miss it by far more than
+    the tolerance."""
+    na, nb = 0.03, 0.11
+    got, D, u = _friction_face(na, nb, along_rows)
+
+    def decay(nf2):
+        ram = G * nf2 * abs(u) / math.pow(2.0, 1.3333330)
+        return (1.0 - D * ram / 2) * 0.4 / (1.0 + D * ram / 2)
+
+    expect = decay(0.5 * (na * na + nb * nb))
+    assert abs(got - expect) <= 1e-15, (got, expect)
+    for wrong in (((na + nb) / 2) ** 2, na * na, nb * nb):
+        assert abs(decay(wrong) - expect) > 1e-7
+    # swapping the two cells gives the same face value
+    got2, _, _ = _friction_face(nb, na, along_rows)
+    assert abs(got2 - expect) <= 1e-15
+
+
+def test_rain_block_map_on_still_water():
+    """Spatial rain (block multiplier map x table): on still water every cell
+    grows by D rr m p with m its block's multiplier; the map is row-major
+    over ceil(rows/b) x ceil(cols/b) blocks of b x b cells, the last blocks
+    ragged.  The expected field is built by repeating the map (no index
+    formula), and a map of ones equals no map bitwise."""
+    R, Cc, b = 11, 13, 4
+    br, bc = (R + b - 1) // b, (Cc + b - 1) // b
+    bmap = (np.arange(br * bc, dtype=np.float64) + 1.0) * 0.25   # 0.25, 0.5, ... distinct
+    z = np.zeros((R, Cc))
+    h = np.full((R, Cc), 0.5)
+    LWL = 0.25
+    table = np.full(4, 7.5)
+    dtrain = 60.0
+    o = O.OracleSim(z, np.ones(z.shape, np.uint8), h, 1.0, 0.5, 0.03, LWL=LWL, forcing_mode=1)
+    o.set_rain(table, dtrain, bmap, b)
+    o.step(1)
+    D = 1.0
+    rr = 7.5 / (dtrain * 1000.0)
+    m = np.repeat(np.repeat(bmap.reshape(br, bc), b, axis=0), b, axis=1)[:R, :Cc]
+    expect = 0.5 + D * rr * m * (1.0 - LWL)
+    got = o.read(O.H)
+    assert np.allclose(got, expect, rtol=0, atol=2e-16), np.abs(got - expect).max()
+    # the transposed map would put other multipliers on at least one cell
+    mt = np.repeat(np.repeat(bmap.reshape(bc, br).T, b, axis=0), b, axis=1)[:R, :Cc]
+    assert np.abs(mt - m).max() > 0
+    assert abs(o.stats()["rain_in"] - D * rr * (1.0 - LWL) * m.sum()) < 1e-15
+    # a map of ones is the uniform rain of the table
+    zr, hr = random_case(29, R, Cc, holes=0.05)
+    act = (zr != -9999.0).astype(np.uint8)
+    u = O.OracleSim(zr, act, hr, 1.0, 0.05, 0.03, LWL=LWL)
+    w = O.OracleSim(zr, act, hr, 1.0, 0.05, 0.03, LWL=LWL)
+    u.set_rain(np.array([4.0, 9.0, 2.0]), 5.0)
+    w.set_rain(np.array([4.0, 9.0, 2.0]), 5.0, np.ones(br * bc), b)
+    u.step(150)
+    w.step(150)
+    for f in (O.H, O.U, O.V):
+        assert np.array_equal(u.read(f), w.read(f))
