@@ -156,7 +156,9 @@ int sw2d_get_stats(sw2d_ctx *ctx, sw2d_stats *out);
 int sw2d_snapshot(sw2d_ctx *ctx, void *buf, size_t *size);
 int sw2d_restore(sw2d_ctx *ctx, const void *buf, size_t size);
 
-/* Device bytes held by a context with these dimensions and precision. */
+/* Device bytes a context with these dimensions and precision needs: its
+ * arrays, the active-cell mask and the peak of the transient staging buffer
+ * (8 B per cell) that uploads and read-backs pass through. */
 size_t sw2d_footprint_bytes(int rows, int cols, int precision, int has_n);
 
 const char *sw2d_last_error(void);

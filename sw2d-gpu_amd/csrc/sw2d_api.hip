@@ -12,10 +12,12 @@ namespace {
 thread_local std::string g_err;
 
 // Dense <-> pitched conversions on the device (see sw2d::launch_pack).
+// dst, src and mask start at row `row0` of their arrays; row0 only places
+// the rows in the datum blocks
 template <typename T>
 __global__ void k_pack(T *__restrict__ dst, const double *__restrict__ src, const unsigned char *__restrict__ mask,
                        int rows, int cols, int P, int total_rows, double pad, int mode,
-                       const float *__restrict__ datum, int datum_cols, double g) {
+                       const float *__restrict__ datum, int datum_cols, double g, int row0) {
   const size_t n = (size_t)total_rows * P;
   for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
     const int i = (int)(k / (size_t)P), j = (int)(k % (size_t)P);
@@ -27,7 +29,8 @@ __global__ void k_pack(T *__restrict__ dst, const double *__restrict__ src, cons
         if (!mask[s]) v = -1.0;
       } else if (mode == sw2d::PACK_BED) {
         if (!mask[s]) v = 0.0;
-        else if (datum) v -= (double)datum[(size_t)(i / sw2d::DATUM_BLOCK) * datum_cols + j / sw2d::DATUM_BLOCK];
+        else if (datum)
+          v -= (double)datum[(size_t)((row0 + i) / sw2d::DATUM_BLOCK) * datum_cols + j / sw2d::DATUM_BLOCK];
       } else if (mode == sw2d::PACK_GN2) {
         v = g * v * v;
       }
@@ -53,14 +56,14 @@ int conv_blocks(size_t n) { return (int)std::max<size_t>(1, std::min<size_t>((n 
 namespace sw2d {
 hipError_t launch_pack(int precision, void *dst, const double *src, const unsigned char *mask, int rows, int cols,
                        int P, int total_rows, double pad, int mode, const float *datum, int datum_cols, double g,
-                       hipStream_t s) {
+                       int row0, hipStream_t s) {
   const int nb = conv_blocks((size_t)total_rows * P);
   if (precision == SW2D_FP64)
     hipLaunchKernelGGL(k_pack<double>, dim3(nb), dim3(256), 0, s, (double *)dst, src, mask, rows, cols, P,
-                       total_rows, pad, mode, datum, datum_cols, g);
+                       total_rows, pad, mode, datum, datum_cols, g, row0);
   else
     hipLaunchKernelGGL(k_pack<float>, dim3(nb), dim3(256), 0, s, (float *)dst, src, mask, rows, cols, P, total_rows,
-                       pad, mode, datum, datum_cols, g);
+                       pad, mode, datum, datum_cols, g, row0);
   return hipGetLastError();
 }
 hipError_t launch_unpack(int precision, double *dst, const void *src, int rows, int cols, int P, int clamp0,
@@ -101,6 +104,9 @@ size_t sw2d_footprint_bytes(int rows, int cols, int precision, int has_n) {
   if (has_n) b += esz * cells;
   b += tiles * 2 * sw2d::ACC_N * sizeof(double);
   b += 4 * (((size_t)rows + 63) / 64) * (((size_t)cols + 63) / 64);
+  // the active-cell mask (one byte per cell), and the transient staging
+  // buffer that uploads and read-backs convert through
+  b += (size_t)rows * cols + sw2d::staging_bytes(rows, cols);
   return b + 65536;
 }
 
@@ -486,6 +492,7 @@ int sw2d_get_dt_history(sw2d_ctx *x, double *out, int capacity, int *n_out) {
 
 int sw2d_step(sw2d_ctx *x, int64_t n_iter) {
   if (!x || n_iter < 0) return fail(SW2D_E_INVALID, "step count");
+  if (x->owner) return fail(SW2D_E_STATE, "a batch scenario is stepped by sw2d_batch_step");
   if (n_iter == 0) return SW2D_OK;
   sw2d::IterLaunch a;
   a.grids = x->d_grid;

@@ -103,6 +103,10 @@ struct sw2d_batch {
   }
 
   int rebuild() {
+    // iterations still in flight read d_grids, d_ctrls and d_work and may be
+    // replaying the graph about to be destroyed: the null-stream copies
+    // below do not order against the batch stream
+    HIPCHK(hipStreamSynchronize(stream));
     live_ids.clear();
     for (size_t id = 0; id < scen.size(); id++)
       if (scen[id]) live_ids.push_back((int)id);
@@ -181,6 +185,7 @@ void sw2d_batch_destroy(sw2d_batch *b) { delete b; }
 int sw2d_batch_add(sw2d_batch *b, const sw2d_grid *grid, const sw2d_params *params, int share_with, int *id_out) {
   if (!b || !grid || !params || !id_out) return fail(SW2D_E_INVALID, "null argument");
   if (params->precision != b->prec) return fail(SW2D_E_INVALID, "scenario precision differs from the batch");
+  HIPCHK(hipStreamSynchronize(b->stream));
   sw2d_ctx *sib = nullptr;
   if (share_with >= 0) {
     if (share_with >= (int)b->scen.size() || !(sib = b->scen[share_with]))
@@ -220,6 +225,7 @@ int sw2d_batch_add(sw2d_batch *b, const sw2d_grid *grid, const sw2d_params *para
     c->ncell = sib->ncell;
     c->datum = sib->datum;
   }
+  c->owner = b->stream;
   b->scen.push_back(c);
   b->share_of.push_back(sib ? (b->share_of[share_with] >= 0 ? b->share_of[share_with] : share_with) : -1);
   b->rows_override.push_back(0);

@@ -106,6 +106,10 @@ struct sw2d_ctx {
   double nodata = 0;
   double dx = 0, dy = 0;
   hipStream_t stream = nullptr;
+  // A batch scenario's work runs on the batch stream: every call on the
+  // context waits for it as well (fetch_ctrl), and the context cannot be
+  // stepped on its own.
+  hipStream_t owner = nullptr;
   std::vector<void *> allocs;
   void *h[2]{}, *U[2]{}, *V[2]{}, *z = nullptr, *ncell = nullptr;
   float *datum = nullptr;
@@ -175,6 +179,7 @@ struct sw2d_ctx {
   }
 
   int fetch_ctrl(Ctrl &c) {
+    if (owner) HIPCHK(hipStreamSynchronize(owner));
     HIPCHK(hipStreamSynchronize(stream));
     HIPCHK(hipMemcpy(&c, d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
     return SW2D_OK;
@@ -201,15 +206,24 @@ struct sw2d_ctx {
     return SW2D_OK;
   }
   // Copy a device array with pitch P into a dense host double array: the
-  // conversion runs on the device (clamp0: negative values read as 0), in a
-  // stream-ordered staging buffer, and one copy brings the result back.
+  // conversion runs on the device (clamp0: negative values read as 0), a
+  // chunk of rows at a time through a stream-ordered staging buffer of at
+  // most sw2d::STAGING_MAX bytes.
   int read_pitched(const void *src, int rows, int cols, double *out, bool clamp0 = false) {
     const size_t n = (size_t)rows * cols;
     if (n == 0) return SW2D_OK;
+    const int cr = sw2d::staging_rows(rows, cols);
     double *d_out = nullptr;
-    HIPCHK(hipMallocAsync((void **)&d_out, n * sizeof(double), stream));
-    hipError_t e = sw2d::launch_unpack(prec, d_out, src, rows, cols, P, clamp0 ? 1 : 0, stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, n * sizeof(double), hipMemcpyDeviceToHost, stream);
+    HIPCHK(hipMallocAsync((void **)&d_out, (size_t)cr * cols * sizeof(double), stream));
+    hipError_t e = hipSuccess;
+    for (int a = 0; a < rows && e == hipSuccess; a += cr) {
+      const int nr = std::min(cr, rows - a);
+      e = sw2d::launch_unpack(prec, d_out, (const char *)src + (size_t)a * P * esz, nr, cols, P, clamp0 ? 1 : 0,
+                              stream);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(out + (size_t)a * cols, d_out, (size_t)nr * cols * sizeof(double),
+                           hipMemcpyDeviceToHost, stream);
+    }
     const hipError_t f = hipFreeAsync(d_out, stream);
     const hipError_t y = hipStreamSynchronize(stream);
     if (e == hipSuccess) e = f != hipSuccess ? f : y;
@@ -226,17 +240,28 @@ struct sw2d_ctx {
   }
   // Write a dense host array into a pitched device array of rows + extra
   // rows; the padding columns and the extra rows hold `pad`.  The array goes
-  // up once, in a stream-ordered staging buffer, and the conversion (mode:
+  // up a chunk of rows at a time, through a stream-ordered staging buffer of
+  // at most sw2d::STAGING_MAX bytes, and the conversion (mode:
   // sw2d::PackMode) runs on the device.
   int pack(void *dst, int rows, int cols, const double *in, int mode, int extra = 0, double pad = 0.0,
            double g = 0.0) {
-    const size_t n = (size_t)rows * cols;
+    const int cr = sw2d::staging_rows(rows, cols);
     double *d_in = nullptr;
-    HIPCHK(hipMallocAsync((void **)&d_in, std::max<size_t>(n, 1) * sizeof(double), stream));
-    hipError_t e = n ? hipMemcpyAsync(d_in, in, n * sizeof(double), hipMemcpyHostToDevice, stream) : hipSuccess;
-    if (e == hipSuccess)
-      e = sw2d::launch_pack(prec, dst, d_in, d_active, rows, cols, P, rows + extra, pad, mode, datum, datum_cols, g,
-                            stream);
+    HIPCHK(hipMallocAsync((void **)&d_in, std::max<size_t>((size_t)cr * cols, 1) * sizeof(double), stream));
+    hipError_t e = hipSuccess;
+    for (int a = 0; a < rows && e == hipSuccess; a += cr) {
+      const int nr = std::min(cr, rows - a);
+      const bool last = a + nr == rows;  // the last chunk also writes the extra rows
+      e = hipMemcpyAsync(d_in, in + (size_t)a * cols, (size_t)nr * cols * sizeof(double), hipMemcpyHostToDevice,
+                         stream);
+      // the masked modes convert cell arrays (cols == C); the mask is not
+      // read otherwise
+      const bool masked = mode == sw2d::PACK_CELLS || mode == sw2d::PACK_BED;
+      if (e == hipSuccess)
+        e = sw2d::launch_pack(prec, (char *)dst + (size_t)a * P * esz, d_in,
+                              masked ? d_active + (size_t)a * cols : nullptr, nr, cols, P, nr + (last ? extra : 0), pad,
+                              mode, datum, datum_cols, g, a, stream);
+    }
     const hipError_t f = hipFreeAsync(d_in, stream);
     const hipError_t y = hipStreamSynchronize(stream);
     if (e == hipSuccess) e = f != hipSuccess ? f : y;

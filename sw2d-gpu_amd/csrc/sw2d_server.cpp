@@ -86,7 +86,12 @@ struct sw2d_server {
       rc = sw2d_batch_scenario(batch, id, &ctx);
       if (!rc && !r.rain.empty())
         rc = sw2d_set_rain(ctx, r.rain.data(), (int)r.rain.size(), r.dtrain, nullptr, 0);
-      if (rc) return rc;
+      if (rc) {
+        // the scenario is in the batch but not admitted: take it out again
+        // (the request stays at the head of the queue)
+        (void)sw2d_batch_remove(batch, id);
+        return rc;
+      }
       r.scen = id;
       r.state = SW2D_REQ_RUNNING;
       r.admit_s = now();

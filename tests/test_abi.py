@@ -119,7 +119,12 @@ def test_malformed_setup_is_rejected(tmp_path):
 
 
 def test_footprint_is_about_32_bytes_per_cell_fp32():
-    # h, U, V ping-pong pairs (24 B), bed (4 B) and Manning raster (4 B)
+    # h, U, V ping-pong pairs (24 B), bed (4 B) and Manning raster (4 B), the
+    # active-cell mask (1 B) and a staging buffer of at most 32 MiB
     b = sw2d.footprint_bytes(8192, 8192, sw2d.FP32, True)
     per = b / (8192 * 8192)
-    assert 32 <= per <= 34
+    assert 33 <= per <= 34.5
+    assert b - sw2d.footprint_bytes(8192, 8192, sw2d.FP32, False) >= 4 * 8192 * 8192
+    # the staging buffer is bounded: doubling the rows adds the arrays only
+    b2 = sw2d.footprint_bytes(16384, 8192, sw2d.FP32, True)
+    assert b2 - b < 34 * 8192 * 8192

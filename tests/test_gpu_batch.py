@@ -135,3 +135,46 @@ def test_graph_replay_equals_single_launches():
     for f in (sw2d.F_H, sw2d.F_U, sw2d.F_V):
         assert np.array_equal(a.read(f), b.read(f))
     assert a.stats()["t"] == b.stats()["t"]
+
+
+def test_rebuild_between_steps_without_sync():
+    """Changing a scenario's segment height while iterations are still in
+    flight (no sync between the steps): the rebuild of the work-list waits
+    for the batch stream, so the state equals an alone run bitwise."""
+    say("case: set_tile_rows between asynchronous batch steps")
+    kw = dict(dt=0.2, manning=0.03, cfl=0.45, precision=sw2d.FP32, forcing_mode=sw2d.FORCING_STEP)
+    b = sw2d.Batch(sw2d.FP32)
+    shapes = [(300, 260), (180, 400)]
+    ids = []
+    for k, shp in enumerate(shapes):
+        z, h, n, rain = scen_inputs(20 + k, shp)
+        sid = b.add(z, h, 2.0, n_cell=n, **kw)
+        b.scenario(sid).set_rain(rain, 300.0)
+        ids.append(sid)
+    b.touch()
+    b.step(150)                      # asynchronous: graph replays in flight
+    b.set_tile_rows(ids[0], 8)
+    b.set_tile_rows(ids[1], 24)
+    b.step(90)
+    b.set_tile_rows(ids[0], 0)
+    b.step(37)
+    for k, shp in enumerate(shapes):
+        z, h, n, rain = scen_inputs(20 + k, shp)
+        a = sw2d.Sim(z, h, 2.0, n_cell=n, **kw)
+        a.set_rain(rain, 300.0)
+        a.step(277)
+        s = b.scenario(ids[k])
+        assert s.stats()["iterations"] == 277
+        for f in (sw2d.F_H, sw2d.F_U, sw2d.F_V):
+            assert np.array_equal(a.read(f), s.read(f)), (k, f)
+
+
+def test_batch_scenario_context_is_not_stepped_alone():
+    say("case: sw2d_step on a batch scenario is refused")
+    b = sw2d.Batch(sw2d.FP32)
+    sid = b.add(*random_case(31, 40, 60)[:2], 2.0, dt=0.05, manning=0.03, precision=sw2d.FP32)
+    with pytest.raises(sw2d.SW2DError) as e:
+        b.scenario(sid).step(1)
+    assert e.value.code == sw2d.E_STATE
+    b.step(3)
+    assert b.status()[1][sid] == 3
