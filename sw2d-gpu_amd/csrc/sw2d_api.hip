@@ -12,67 +12,88 @@ namespace {
 thread_local std::string g_err;
 
 // Dense <-> pitched conversions on the device (see sw2d::launch_pack).
-// dst, src and mask start at row `row0` of their arrays; row0 only places
-// the rows in the datum blocks
+// rows x cols values of a dense double array into a pitched array (row
+// stride `pitch`); dst, src and mask start at row `row0` of their arrays,
+// which places the rows in the bed datum blocks
 template <typename T>
 __global__ void k_pack(T *__restrict__ dst, const double *__restrict__ src, const unsigned char *__restrict__ mask,
-                       int rows, int cols, int P, int total_rows, double pad, int mode,
-                       const float *__restrict__ datum, int datum_cols, double g, int row0) {
-  const size_t n = (size_t)total_rows * P;
-  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
-    const int i = (int)(k / (size_t)P), j = (int)(k % (size_t)P);
-    double v = pad;
-    if (i < rows && j < cols) {
-      const size_t s = (size_t)i * cols + j;
-      v = src[s];
-      if (mode == sw2d::PACK_CELLS) {
-        if (!mask[s]) v = -1.0;
-      } else if (mode == sw2d::PACK_BED) {
-        if (!mask[s]) v = 0.0;
-        else if (datum)
-          v -= (double)datum[(size_t)((row0 + i) / sw2d::DATUM_BLOCK) * datum_cols + j / sw2d::DATUM_BLOCK];
-      } else if (mode == sw2d::PACK_GN2) {
-        v = g * v * v;
-      }
+                       int rows, int cols, size_t pitch, int mode, const float *__restrict__ datum, int datum_cols,
+                       double g, int row0) {
+  const size_t n = (size_t)rows * cols;
+  for (size_t s = blockIdx.x * (size_t)blockDim.x + threadIdx.x; s < n; s += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(s / (size_t)cols), j = (int)(s % (size_t)cols);
+    double v = src[s];
+    if (mode == sw2d::PACK_CELLS) {
+      if (!mask[s]) v = -1.0;
+    } else if (mode == sw2d::PACK_BED) {
+      if (!mask[s]) v = 0.0;
+      else if (datum)
+        v -= (double)datum[(size_t)((row0 + i) / sw2d::DATUM_BLOCK) * datum_cols + j / sw2d::DATUM_BLOCK];
+    } else if (mode == sw2d::PACK_GN2) {
+      v = g * v * v;
     }
-    dst[k] = (T)v;
+    dst[(size_t)i * pitch + j] = (T)v;
+  }
+}
+
+// the padding pattern of the state block over `rows` block rows of six
+// pieces of P elements (see sw2d::STATE_ARRAYS): h pieces -1, U and V pieces
+// 0, and the last four columns of each piece the value of the next piece
+template <typename T>
+__global__ void k_slab_init(T *__restrict__ dst, size_t rows, int P) {
+  const size_t RS = (size_t)sw2d::STATE_ARRAYS * P, n = rows * RS;
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
+    const int e = (int)(k % RS), piece = e / P, j = e % P;
+    const int p = j >= P - 4 ? (piece + 1) % sw2d::STATE_ARRAYS : piece;
+    dst[k] = p / 2 == 0 ? (T)-1 : (T)0;
   }
 }
 
 template <typename T>
-__global__ void k_unpack(double *__restrict__ dst, const T *__restrict__ src, int rows, int cols, int P, int clamp0) {
+__global__ void k_unpack(double *__restrict__ dst, const T *__restrict__ src, int rows, int cols, size_t pitch,
+                         int clamp0) {
   const size_t n = (size_t)rows * cols;
   for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
     const int i = (int)(k / (size_t)cols), j = (int)(k % (size_t)cols);
-    double v = (double)src[(size_t)i * P + j];
+    double v = (double)src[(size_t)i * pitch + j];
     if (clamp0 && v < 0.0) v = 0.0;
     dst[k] = v;
   }
 }
 
 int conv_blocks(size_t n) { return (int)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 8192)); }
+
 }  // namespace
 
 namespace sw2d {
 hipError_t launch_pack(int precision, void *dst, const double *src, const unsigned char *mask, int rows, int cols,
-                       int P, int total_rows, double pad, int mode, const float *datum, int datum_cols, double g,
-                       int row0, hipStream_t s) {
-  const int nb = conv_blocks((size_t)total_rows * P);
+                       size_t pitch, int mode, const float *datum, int datum_cols, double g, int row0,
+                       hipStream_t s) {
+  const int nb = conv_blocks((size_t)rows * cols);
   if (precision == SW2D_FP64)
-    hipLaunchKernelGGL(k_pack<double>, dim3(nb), dim3(256), 0, s, (double *)dst, src, mask, rows, cols, P,
-                       total_rows, pad, mode, datum, datum_cols, g, row0);
+    hipLaunchKernelGGL(k_pack<double>, dim3(nb), dim3(256), 0, s, (double *)dst, src, mask, rows, cols, pitch, mode,
+                       datum, datum_cols, g, row0);
   else
-    hipLaunchKernelGGL(k_pack<float>, dim3(nb), dim3(256), 0, s, (float *)dst, src, mask, rows, cols, P, total_rows,
-                       pad, mode, datum, datum_cols, g, row0);
+    hipLaunchKernelGGL(k_pack<float>, dim3(nb), dim3(256), 0, s, (float *)dst, src, mask, rows, cols, pitch, mode,
+                       datum, datum_cols, g, row0);
   return hipGetLastError();
 }
-hipError_t launch_unpack(int precision, double *dst, const void *src, int rows, int cols, int P, int clamp0,
+hipError_t launch_slab_init(int precision, void *dst, size_t rows, int P, hipStream_t s) {
+  const int nb = conv_blocks(rows * (size_t)STATE_ARRAYS * P);
+  if (precision == SW2D_FP64)
+    hipLaunchKernelGGL(k_slab_init<double>, dim3(nb), dim3(256), 0, s, (double *)dst, rows, P);
+  else
+    hipLaunchKernelGGL(k_slab_init<float>, dim3(nb), dim3(256), 0, s, (float *)dst, rows, P);
+  return hipGetLastError();
+}
+hipError_t launch_unpack(int precision, double *dst, const void *src, int rows, int cols, size_t pitch, int clamp0,
                          hipStream_t s) {
   const int nb = conv_blocks((size_t)rows * cols);
   if (precision == SW2D_FP64)
-    hipLaunchKernelGGL(k_unpack<double>, dim3(nb), dim3(256), 0, s, dst, (const double *)src, rows, cols, P, clamp0);
+    hipLaunchKernelGGL(k_unpack<double>, dim3(nb), dim3(256), 0, s, dst, (const double *)src, rows, cols, pitch,
+                       clamp0);
   else
-    hipLaunchKernelGGL(k_unpack<float>, dim3(nb), dim3(256), 0, s, dst, (const float *)src, rows, cols, P, clamp0);
+    hipLaunchKernelGGL(k_unpack<float>, dim3(nb), dim3(256), 0, s, dst, (const float *)src, rows, cols, pitch, clamp0);
   return hipGetLastError();
 }
 }  // namespace sw2d
@@ -98,10 +119,9 @@ size_t sw2d_footprint_bytes(int rows, int cols, int precision, int has_n) {
   const size_t P = sw2d::state_pitch(cols);
   const size_t sw = (size_t)sw2d::strip_width(precision == SW2D_FP64);
   const size_t tiles = ((size_t)cols + sw - 1) / sw * (((size_t)rows + 3) / 4);
-  size_t cells = (size_t)rows * P, hv = (size_t)(rows + 1) * P, faces = (size_t)(rows + 2) * P;
-  // h, U, V ping-pong pairs (with their padding rows) and the bed
-  size_t b = esz * (2 * hv + 2 * faces + 2 * hv + cells);
-  if (has_n) b += esz * cells;
+  // the state block (h, U, V ping-pong pairs) and the static block (bed and
+  // raster), with their padding rows
+  size_t b = esz * sw2d::slab_rows(rows) * P * (sw2d::STATE_ARRAYS + (has_n ? 2 : 1)) + 4 * esz;
   b += tiles * 2 * sw2d::ACC_N * sizeof(double);
   b += 4 * (((size_t)rows + 63) / 64) * (((size_t)cols + 63) / 64);
   // the active-cell mask (one byte per cell), and the transient staging
@@ -172,21 +192,29 @@ int sw2d_create(const sw2d_grid *grid, const sw2d_params *params, sw2d_ctx **out
   if (hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking) != hipSuccess)
     return bad(fail(SW2D_E_DEVICE, "hipStreamCreate failed"));
 
-  // h and V: R rows + 1 padding row; U: R+1 rows + 1 padding row
-  const size_t cells = (size_t)R * x->P, hv = (size_t)(R + 1) * x->P, faces = (size_t)(R + 2) * x->P;
-  // The step reads every pitched row from column j0 - 4 of its strip (j0 a
-  // multiple of the strip width) and writes from column j0.  The state rows
-  // start on a 256-byte boundary, so the written row pieces are whole 32-byte
-  // sectors in FP32; the bed and raster are only read, and start
-  // STATIC_SHIFT_COLS elements into their block, so their row reads are.
-  constexpr size_t STATE_SHIFT_COLS = 0, STATIC_SHIFT_COLS = 4;
-  int rc = 0;
-  for (int b = 0; b < 2 && !rc; b++) {
-    rc = rc ? rc : x->alloc(&x->h[b], hv * x->esz, STATE_SHIFT_COLS * x->esz);
-    rc = rc ? rc : x->alloc(&x->U[b], faces * x->esz, STATE_SHIFT_COLS * x->esz);
-    rc = rc ? rc : x->alloc(&x->V[b], hv * x->esz, STATE_SHIFT_COLS * x->esz);
+  // Two blocks (see sw2d::STATE_ARRAYS): the state arrays interleaved by row
+  // (row stride RS = 6 P) and the bed and raster (RSS = P or 2 P), each with
+  // PAD_ROWS_ABOVE rows above row 0.  The step reads every row piece from
+  // column j0 - 4 of its strip (j0 a multiple of the strip width) and writes
+  // from column j0.  State rows start on a 256-byte boundary, so the written
+  // row pieces are whole 32-byte sectors in FP32; the static block is only
+  // read, and starts STATIC_SHIFT_COLS elements further into its allocation,
+  // so its row reads are.
+  x->RS = sw2d::STATE_ARRAYS * x->P;
+  x->RSS = (grid->n_cell ? 2 : 1) * x->P;
+  const size_t srows = sw2d::slab_rows(R);
+  constexpr size_t STATIC_SHIFT_COLS = 4;
+  int rc = x->alloc(&x->slab, srows * x->RS * x->esz, (size_t)sw2d::PAD_ROWS_ABOVE * x->RS * x->esz);
+  rc = rc ? rc : x->alloc(&x->z, srows * x->RSS * x->esz,
+                          ((size_t)sw2d::PAD_ROWS_ABOVE * x->RSS + STATIC_SHIFT_COLS) * x->esz);
+  if (!rc) {
+    for (int b = 0; b < 2; b++) {
+      x->h[b] = (char *)x->slab + (size_t)sw2d::state_piece(0, b) * x->P * x->esz;
+      x->U[b] = (char *)x->slab + (size_t)sw2d::state_piece(1, b) * x->P * x->esz;
+      x->V[b] = (char *)x->slab + (size_t)sw2d::state_piece(2, b) * x->P * x->esz;
+    }
+    if (grid->n_cell) x->ncell = (char *)x->z + (size_t)x->P * x->esz;
   }
-  rc = rc ? rc : x->alloc(&x->z, cells * x->esz, STATIC_SHIFT_COLS * x->esz);
   rc = rc ? rc : x->alloc((void **)&x->acc, (size_t)x->n_acc * 2 * sw2d::ACC_N * sizeof(double));
   x->vol_blocks = x->blocks_for((long long)R * C);
   rc = rc ? rc : x->alloc((void **)&x->partial, (size_t)x->vol_blocks * 2 * sizeof(double));
@@ -221,25 +249,29 @@ int sw2d_create(const sw2d_grid *grid, const sw2d_params *params, sw2d_ctx **out
     if (hipMemcpy(x->datum, x->datum_host.data(), x->datum_host.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
       return bad(fail(SW2D_E_DEVICE, "datum upload"));
   }
-  // the bed: 0 on inactive cells; FP32: relative to its block's datum
-  rc = x->pack(x->z, R, C, grid->z, sw2d::PACK_BED);
+  // the static block: 0 in the padding; the bed 0 on inactive cells and,
+  // in FP32, relative to its block's datum; the raster (FP32: g n^2)
+  if (hipMemsetAsync((char *)x->z - ((size_t)sw2d::PAD_ROWS_ABOVE * x->RSS + STATIC_SHIFT_COLS) * x->esz, 0,
+                     srows * x->RSS * x->esz, x->stream) != hipSuccess)
+    return bad(fail(SW2D_E_DEVICE, "bed clear"));
+  rc = x->pack(x->z, R, C, grid->z, sw2d::PACK_BED, x->RSS);
   if (rc) return bad(rc);
   if (grid->n_cell) {
-    rc = x->alloc(&x->ncell, cells * x->esz, STATIC_SHIFT_COLS * x->esz);
-    if (rc) return bad(rc);
-    // the FP32 form stores g n^2
-    rc = x->pack(x->ncell, R, C, grid->n_cell, x->prec == SW2D_FP64 ? sw2d::PACK_PLAIN : sw2d::PACK_GN2, 0, 0.0,
+    rc = x->pack(x->ncell, R, C, grid->n_cell, x->prec == SW2D_FP64 ? sw2d::PACK_PLAIN : sw2d::PACK_GN2, x->RSS,
                  params->g);
     if (rc) return bad(rc);
   }
+  // the state block: padding values everywhere (written once: the kernels
+  // never write the padding), U = V = 0, then the initial depth into both
+  // ping-pong buffers
+  if (sw2d::launch_slab_init(x->prec, (char *)x->slab - (size_t)sw2d::PAD_ROWS_ABOVE * x->RS * x->esz,
+                             srows, x->P, x->stream) != hipSuccess)
+    return bad(fail(SW2D_E_DEVICE, "state clear"));
   rc = x->write_h(x->h[0], grid->h0);
+  rc = rc ? rc : x->write_h(x->h[1], grid->h0);
   if (rc) return bad(rc);
-  for (int b = 0; b < 2; b++) {
-    if (hipMemset(x->U[b], 0, faces * x->esz) != hipSuccess || hipMemset(x->V[b], 0, hv * x->esz) != hipSuccess)
-      return bad(fail(SW2D_E_DEVICE, "state clear"));
-  }
-  if (hipMemcpy(x->h[1], x->h[0], hv * x->esz, hipMemcpyDeviceToDevice) != hipSuccess ||
-      hipMemset(x->acc, 0, (size_t)x->n_acc * 2 * sw2d::ACC_N * sizeof(double)) != hipSuccess)
+  if (hipMemsetAsync(x->acc, 0, (size_t)x->n_acc * 2 * sw2d::ACC_N * sizeof(double), x->stream) != hipSuccess ||
+      hipStreamSynchronize(x->stream) != hipSuccess)
     return bad(fail(SW2D_E_DEVICE, "state init"));
 
   if (hipMemcpy(x->d_ctrl_list, &x->d_ctrl, sizeof(Ctrl *), hipMemcpyHostToDevice) != hipSuccess)
@@ -263,6 +295,8 @@ int sw2d_create(const sw2d_grid *grid, const sw2d_params *params, sw2d_ctx **out
     G.R = R;
     G.C = C;
     G.P = x->P;
+    G.RS = x->RS;
+    G.RSS = x->RSS;
     G.tiles_x = x->tiles_x;
     G.tiles_y = x->tiles_y;
     G.seg_rows = x->TH;
@@ -570,11 +604,11 @@ int sw2d_read(sw2d_ctx *x, int field, double *out) {
   const int p = c.parity, R = x->R, C = x->C;
   switch (field) {
     case SW2D_FIELD_H:  // inactive cells (-1) read as 0
-      return x->read_pitched(x->h[p], R, C, out, true);
+      return x->read_pitched(x->h[p], R, C, out, x->RS, true);
     case SW2D_FIELD_U:
-      return x->read_pitched(x->U[p], R + 1, C, out);
+      return x->read_pitched(x->U[p], R + 1, C, out, x->RS);
     case SW2D_FIELD_V:
-      return x->read_pitched(x->V[p], R, C + 1, out);
+      return x->read_pitched(x->V[p], R, C + 1, out, x->RS);
     case SW2D_FIELD_UU:
     case SW2D_FIELD_VV: {
       if (!x->d_uu) {
@@ -589,7 +623,7 @@ int sw2d_read(sw2d_ctx *x, int field, double *out) {
       return SW2D_OK;
     }
     case SW2D_FIELD_Z:
-      rc = x->read_pitched(x->z, R, C, out);
+      rc = x->read_pitched(x->z, R, C, out, x->RSS);
       if (!rc && x->prec == SW2D_FP32)
         for (int i = 0; i < R; i++)
           for (int j = 0; j < C; j++)
@@ -684,12 +718,9 @@ int sw2d_snapshot(sw2d_ctx *x, void *buf, size_t *size) {
   std::memcpy(o, &hd, sizeof(hd));
   o += sizeof(hd);
   auto grab = [&](void *src, int rows, int cols) -> int {
-    std::vector<unsigned char> tmp((size_t)rows * x->P * x->esz);
-    HIPCHK(hipMemcpy(tmp.data(), src, tmp.size(), hipMemcpyDeviceToHost));
-    for (int i = 0; i < rows; i++) {
-      std::memcpy(o, tmp.data() + (size_t)i * x->P * x->esz, (size_t)cols * x->esz);
-      o += (size_t)cols * x->esz;
-    }
+    HIPCHK(hipMemcpy2D(o, (size_t)cols * x->esz, src, (size_t)x->RS * x->esz, (size_t)cols * x->esz, rows,
+                       hipMemcpyDeviceToHost));
+    o += (size_t)rows * cols * x->esz;
     return SW2D_OK;
   };
   rc = grab(x->h[c.parity], x->R, x->C);
@@ -716,20 +747,17 @@ int sw2d_restore(sw2d_ctx *x, const void *buf, size_t size) {
   int rc = x->fetch_ctrl(c);
   if (rc) return rc;
   const unsigned char *in = (const unsigned char *)buf + sizeof(hd);
-  // the padding of the state arrays (one extra row, columns beyond the
-  // grid) holds the values outside the grid: -1 for h, 0 for U and V
-  auto put = [&](void *dst, int rows, int cols, double pad) -> int {
-    std::vector<unsigned char> tmp = x->pad_rows(rows + 1, pad);
-    for (int i = 0; i < rows; i++) {
-      std::memcpy(tmp.data() + (size_t)i * x->P * x->esz, in, (size_t)cols * x->esz);
-      in += (size_t)cols * x->esz;
-    }
-    HIPCHK(hipMemcpy(dst, tmp.data(), tmp.size(), hipMemcpyHostToDevice));
+  // only the grid's rows and columns: the padding holds the values outside
+  // the grid from creation on
+  auto put = [&](void *dst, int rows, int cols) -> int {
+    HIPCHK(hipMemcpy2D(dst, (size_t)x->RS * x->esz, in, (size_t)cols * x->esz, (size_t)cols * x->esz, rows,
+                       hipMemcpyHostToDevice));
+    in += (size_t)rows * cols * x->esz;
     return SW2D_OK;
   };
-  rc = put(x->h[0], x->R, x->C, -1.0);
-  rc = rc ? rc : put(x->U[0], x->R + 1, x->C, 0.0);
-  rc = rc ? rc : put(x->V[0], x->R, x->C + 1, 0.0);
+  rc = put(x->h[0], x->R, x->C);
+  rc = rc ? rc : put(x->U[0], x->R + 1, x->C);
+  rc = rc ? rc : put(x->V[0], x->R, x->C + 1);
   if (rc) return rc;
   HIPCHK(hipMemcpy(x->acc, in, nacc * sizeof(double), hipMemcpyHostToDevice));
   c.parity = 0;

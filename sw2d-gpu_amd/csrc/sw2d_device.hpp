@@ -2,17 +2,19 @@
 // and the host runtime of the MI355X SW2D core.
 //
 // Memory layout in HBM (one scenario):
-//   all arrays row-major with one pitch P >= cols+2 (multiple of 64 elements)
-//   h[2]      rows+1 x P   depth, ping-pong; inactive cells hold -1
-//   U[2]      rows+2 x P   discharge on U faces (row i = face north of cell i)
-//   V[2]      rows+1 x P   discharge on V faces (column j = face west of j,
+//   all arrays row-major, row pieces of P >= cols+9 elements (a multiple of
+//   64), interleaved by row in two blocks (see STATE_ARRAYS), with padding
+//   rows above and below the grid
+//   h[2]      rows x P     depth, ping-pong; inactive cells hold -1
+//   U[2]      rows+1 x P   discharge on U faces (row i = face north of cell i)
+//   V[2]      rows x P     discharge on V faces (column j = face west of j,
 //                          column cols = east boundary face)
 //   z         rows   x P   bed (FP32 form: minus a per-64x64-block datum)
 //   ncell     rows   x P   optional Manning raster (FP32 form: g*n^2)
-// The last row of h, U and V and their columns beyond the grid (from cols,
-// or cols+1 for V) are padding that holds the values outside the grid:
-// h = -1 (no cell), U = V = 0 (closed).  The step kernel reads neighbours
-// outside the grid there instead of masking them.
+// The padding rows and the columns beyond the grid (from cols, or cols+1 for
+// V) hold the values outside the grid: h = -1 (no cell), U = V = 0 (closed),
+// bed and raster 0.  The step kernel reads neighbours outside the grid there
+// instead of masking them.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -34,9 +36,29 @@ constexpr int ACC_N = 5;            // rain, source, evap, outflow, clamp
 constexpr int SMAX_SLOTS = 64;      // CFL maximum: workgroups spread their atomics
 constexpr int EDGE_N = 1, EDGE_S = 2, EDGE_W = 4, EDGE_E = 8;
 
-// pitch of every pitched array: the V east face column cols and at least one
-// padding column beyond it, rounded to 64 elements
-__host__ __device__ inline size_t state_pitch(int cols) { return ((size_t)cols + 2 + 63) / 64 * 64; }
+// Width P of one array's row piece: the V east face column cols, then at
+// least eight padding columns, rounded to 64 elements.
+__host__ __device__ inline size_t state_pitch(int cols) { return ((size_t)cols + 9 + 63) / 64 * 64; }
+
+// The six state arrays live in one block, interleaved by row: row r holds
+// the row pieces of h0 h1 U0 U1 V0 V1 (STATE_ARRAYS pieces of P elements, row
+// stride RS = 6 P); bed and raster likewise share a second block (row stride
+// P, or 2 P with a raster).  Each block holds PAD_ROWS_ABOVE rows above row 0
+// and rows_after(rows) rows from row 0 on: the grid rows, then padding rows.
+// The padding rows and the columns beyond the grid hold the values outside
+// the grid (h = -1, U = V = 0, bed and raster 0), except the last four
+// columns of each piece, which hold those of the NEXT piece (cyclically, V1
+// before the next row's h0): the step kernel reads a row from column j0 - 4
+// of its strip, so on the first strip columns -4 .. -1 of a piece are the
+// last four columns of the piece before it.  A lane whose column lies beyond
+// cols + 4 reads the following piece's first columns; such a lane is more
+// than the stencil's three columns away from every output column.
+constexpr int STATE_ARRAYS = 6;
+constexpr int PAD_ROWS_ABOVE = 5, PAD_ROWS_BELOW = 6;
+__host__ __device__ inline size_t rows_after(int rows) { return (size_t)rows + PAD_ROWS_BELOW; }
+__host__ __device__ inline size_t slab_rows(int rows) { return PAD_ROWS_ABOVE + rows_after(rows); }
+// piece of field f (0 = h, 1 = U, 2 = V) in ping-pong buffer b
+__host__ __device__ inline int state_piece(int f, int b) { return 2 * f + b; }
 constexpr int ERR_NONFINITE = 1, ERR_TOO_MANY_SRC = 2;
 
 // Per-scenario control block, resident in device memory.  The control
@@ -83,6 +105,7 @@ struct Tile {
 template <typename T>
 struct Grid {
   int R, C, P;
+  int RS, RSS;                   // row strides of the state and static blocks (elements)
   int tiles_x, tiles_y;          // strip_width-column strips, row segments
   int seg_rows;                  // rows per segment when no work-list is given
   int edge_open;

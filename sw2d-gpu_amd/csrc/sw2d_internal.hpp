@@ -101,6 +101,7 @@ struct sw2d_ctx {
   int prec = 0;
   size_t esz = 8;
   int R = 0, C = 0, P = 0, TH = 0, tiles_x = 0, tiles_y = 0, n_tiles = 0;
+  int RS = 0, RSS = 0;  // row strides of the state and static blocks (elements)
   int n_acc = 0;  // accumulator slots: enough for the shortest segment (4 rows)
   sw2d_params prm{};
   double nodata = 0;
@@ -111,6 +112,9 @@ struct sw2d_ctx {
   // stepped on its own.
   hipStream_t owner = nullptr;
   std::vector<void *> allocs;
+  void *slab = nullptr;  // the state block (row 0 of its first piece)
+  // row 0 of each state array inside the state block; the bed (its block's
+  // allocation) and the raster inside the static block
   void *h[2]{}, *U[2]{}, *V[2]{}, *z = nullptr, *ncell = nullptr;
   float *datum = nullptr;
   void *rain_map = nullptr;  // rain multiplier map, in the state precision
@@ -205,11 +209,11 @@ struct sw2d_ctx {
     HIPCHK(hipStreamSynchronize(stream));
     return SW2D_OK;
   }
-  // Copy a device array with pitch P into a dense host double array: the
-  // conversion runs on the device (clamp0: negative values read as 0), a
-  // chunk of rows at a time through a stream-ordered staging buffer of at
-  // most sw2d::STAGING_MAX bytes.
-  int read_pitched(const void *src, int rows, int cols, double *out, bool clamp0 = false) {
+  // Copy a device array (row stride `pitch` elements) into a dense host
+  // double array: the conversion runs on the device (clamp0: negative values
+  // read as 0), a chunk of rows at a time through a stream-ordered staging
+  // buffer of at most sw2d::STAGING_MAX bytes.
+  int read_pitched(const void *src, int rows, int cols, double *out, int pitch, bool clamp0 = false) {
     const size_t n = (size_t)rows * cols;
     if (n == 0) return SW2D_OK;
     const int cr = sw2d::staging_rows(rows, cols);
@@ -218,8 +222,8 @@ struct sw2d_ctx {
     hipError_t e = hipSuccess;
     for (int a = 0; a < rows && e == hipSuccess; a += cr) {
       const int nr = std::min(cr, rows - a);
-      e = sw2d::launch_unpack(prec, d_out, (const char *)src + (size_t)a * P * esz, nr, cols, P, clamp0 ? 1 : 0,
-                              stream);
+      e = sw2d::launch_unpack(prec, d_out, (const char *)src + (size_t)a * pitch * esz, nr, cols, (size_t)pitch,
+                              clamp0 ? 1 : 0, stream);
       if (e == hipSuccess)
         e = hipMemcpyAsync(out + (size_t)a * cols, d_out, (size_t)nr * cols * sizeof(double),
                            hipMemcpyDeviceToHost, stream);
@@ -230,37 +234,27 @@ struct sw2d_ctx {
     if (e != hipSuccess) return fail(SW2D_E_DEVICE, std::string("read: ") + hipGetErrorString(e));
     return SW2D_OK;
   }
-  // host staging buffer of `rows` pitched rows, every element `pad` (state
-  // precision; snapshot restore copies raw rows into it)
-  std::vector<unsigned char> pad_rows(int rows, double pad) const {
-    std::vector<unsigned char> buf((size_t)rows * P * esz);
-    if (prec == SW2D_FP64) std::fill((double *)buf.data(), (double *)buf.data() + (size_t)rows * P, pad);
-    else std::fill((float *)buf.data(), (float *)buf.data() + (size_t)rows * P, (float)pad);
-    return buf;
-  }
-  // Write a dense host array into a pitched device array of rows + extra
-  // rows; the padding columns and the extra rows hold `pad`.  The array goes
-  // up a chunk of rows at a time, through a stream-ordered staging buffer of
-  // at most sw2d::STAGING_MAX bytes, and the conversion (mode:
-  // sw2d::PackMode) runs on the device.
-  int pack(void *dst, int rows, int cols, const double *in, int mode, int extra = 0, double pad = 0.0,
-           double g = 0.0) {
+  // Write a dense host array (rows x cols) into a device array with row
+  // stride `pitch` elements; only the grid's values are written (the padding
+  // is set once at creation).  The array goes up a chunk of rows at a time,
+  // through a stream-ordered staging buffer of at most sw2d::STAGING_MAX
+  // bytes, and the conversion (mode: sw2d::PackMode) runs on the device.
+  int pack(void *dst, int rows, int cols, const double *in, int mode, int pitch, double g = 0.0) {
     const int cr = sw2d::staging_rows(rows, cols);
     double *d_in = nullptr;
     HIPCHK(hipMallocAsync((void **)&d_in, std::max<size_t>((size_t)cr * cols, 1) * sizeof(double), stream));
     hipError_t e = hipSuccess;
     for (int a = 0; a < rows && e == hipSuccess; a += cr) {
       const int nr = std::min(cr, rows - a);
-      const bool last = a + nr == rows;  // the last chunk also writes the extra rows
       e = hipMemcpyAsync(d_in, in + (size_t)a * cols, (size_t)nr * cols * sizeof(double), hipMemcpyHostToDevice,
                          stream);
       // the masked modes convert cell arrays (cols == C); the mask is not
       // read otherwise
       const bool masked = mode == sw2d::PACK_CELLS || mode == sw2d::PACK_BED;
       if (e == hipSuccess)
-        e = sw2d::launch_pack(prec, (char *)dst + (size_t)a * P * esz, d_in,
-                              masked ? d_active + (size_t)a * cols : nullptr, nr, cols, P, nr + (last ? extra : 0), pad,
-                              mode, datum, datum_cols, g, a, stream);
+        e = sw2d::launch_pack(prec, (char *)dst + (size_t)a * pitch * esz, d_in,
+                              masked ? d_active + (size_t)a * cols : nullptr, nr, cols, (size_t)pitch, mode, datum,
+                              datum_cols, g, a, stream);
     }
     const hipError_t f = hipFreeAsync(d_in, stream);
     const hipError_t y = hipStreamSynchronize(stream);
@@ -268,18 +262,9 @@ struct sw2d_ctx {
     if (e != hipSuccess) return fail(SW2D_E_DEVICE, std::string("write: ") + hipGetErrorString(e));
     return SW2D_OK;
   }
-  // `cells` marks cell state: inactive cells are written as -1
-  int write_pitched(void *dst, int rows, int cols, const double *in, bool cells, int extra = 0,
-                    double pad = 0.0) {
-    return pack(dst, rows, cols, in, cells ? sw2d::PACK_CELLS : sw2d::PACK_PLAIN, extra, pad);
-  }
-  // The state arrays carry the values outside the grid in their padding, so
-  // the step kernel reads a neighbour outside the grid without masking:
-  // h has one extra row (R) and its padding columns (C .. P-1) all at -1;
-  // U (rows 0 .. R real) has an extra row R+1 and V (columns 0 .. C real) an
-  // extra row R, their padding at 0.
-  int write_h(void *dst, const double *in) { return write_pitched(dst, R, C, in, true, 1, -1.0); }
-  int write_U(void *dst, const double *in) { return write_pitched(dst, R + 1, C, in, false, 1, 0.0); }
-  int write_V(void *dst, const double *in) { return write_pitched(dst, R, C + 1, in, false, 1, 0.0); }
+  // The state arrays: the grid's values of h (inactive cells as -1), U and V.
+  int write_h(void *dst, const double *in) { return pack(dst, R, C, in, sw2d::PACK_CELLS, RS); }
+  int write_U(void *dst, const double *in) { return pack(dst, R + 1, C, in, sw2d::PACK_PLAIN, RS); }
+  int write_V(void *dst, const double *in) { return pack(dst, R, C + 1, in, sw2d::PACK_PLAIN, RS); }
 };
 

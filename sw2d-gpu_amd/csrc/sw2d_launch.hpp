@@ -108,22 +108,24 @@ inline int staging_rows(int rows, int cols) {
 inline size_t staging_bytes(int rows, int cols) {
   return (size_t)staging_rows(rows, cols) * (size_t)cols * sizeof(double);
 }
-// pack: element (i, j) of a rows x cols array goes to row i, column j of the
-// pitched array (pitch P, total_rows rows); the padding columns and rows
-// past `rows` hold `pad`.  Modes: PACK_PLAIN; PACK_CELLS (cell state:
-// inactive cells of `mask` are written as -1); PACK_BED (0 on inactive
-// cells; FP32: minus the cell's block datum); PACK_GN2 (g n^2).  The host
-// restatement of each mode rounds alike (the same double operations, then
-// one conversion to the state precision).  dst, src and mask point at row
-// row0 of their arrays (a chunk of rows); row0 places the rows in the bed
-// datum blocks.
+// pack: element (i, j) of a rows x cols dense array goes to row i, column j
+// of a pitched array (row stride `pitch` elements); nothing else is written.
+// Modes: PACK_PLAIN; PACK_CELLS (cell state: inactive cells of `mask` are
+// written as -1); PACK_BED (0 on inactive cells; FP32: minus the cell's block
+// datum); PACK_GN2 (g n^2).  The host restatement of each mode rounds alike
+// (the same double operations, then one conversion to the state precision).
+// dst, src and mask point at row row0 of their arrays (a chunk of rows);
+// row0 places the rows in the bed datum blocks.
 enum PackMode { PACK_PLAIN = 0, PACK_CELLS = 1, PACK_BED = 2, PACK_GN2 = 3 };
 hipError_t launch_pack(int precision, void *dst, const double *src, const unsigned char *mask, int rows, int cols,
-                       int P, int total_rows, double pad, int mode, const float *datum, int datum_cols, double g,
-                       int row0, hipStream_t s);
+                       size_t pitch, int mode, const float *datum, int datum_cols, double g, int row0,
+                       hipStream_t s);
+// the padding pattern of the state block (see STATE_ARRAYS) over `rows` rows
+// of six pieces of P elements, from dst on
+hipError_t launch_slab_init(int precision, void *dst, size_t rows, int P, hipStream_t s);
 // unpack: the rows x cols part of a pitched array into a dense double array;
 // clamp0 turns negative values (the -1 of inactive cells) into 0.
-hipError_t launch_unpack(int precision, double *dst, const void *src, int rows, int cols, int P, int clamp0,
+hipError_t launch_unpack(int precision, double *dst, const void *src, int rows, int cols, size_t pitch, int clamp0,
                          hipStream_t s);
 
 }  // namespace sw2d
