@@ -44,9 +44,32 @@ def traffic_per_launch(size, precision):
         return None
 
 
-def cpu_baseline(w, h, U, V, seconds=15.0, max_iters=40):
+def cpu_model():
+    """The host CPU's model name (lscpu) and the number of online CPUs."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    if model is None:
+        try:
+            import subprocess
+            out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+            for ln in out.splitlines():
+                if ln.startswith("Model name"):
+                    model = ln.split(":", 1)[1].strip()
+        except (OSError, subprocess.SubprocessError):
+            pass
+    return model, os.cpu_count()
+
+
+def cpu_baseline(w, h, U, V, seconds=20.0, max_iters=200):
     """FP64 OpenMP oracle on the same grid and state (test infrastructure,
-    timed as the CPU baseline)."""
+    timed as the CPU baseline): a bounded sample of about 20 s of CPU work."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py as O
     p = w["params"]
@@ -67,9 +90,49 @@ def cpu_baseline(w, h, U, V, seconds=15.0, max_iters=40):
     dt = time.perf_counter() - t0
     cells = w["z"].size
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    model, ncpu = cpu_model()
     return dict(value=cells * n / dt / 1e9, unit="Gcell-updates/s", cores=threads, kind="port",
                 sample="%d FP64 iterations of the same %dx%d grid from the GPU's warmed-up state "
-                       "(%.1f s)" % (n, w["z"].shape[0], w["z"].shape[1], dt))
+                       "(%.1f s)" % (n, w["z"].shape[0], w["z"].shape[1], dt),
+                cpu_model=model, nproc=ncpu, omp_num_threads=os.environ.get("OMP_NUM_THREADS"))
+
+
+def sustained_run(w, cells, precision):
+    """The whole configured run (50 + 19,950 iterations from the release) on
+    a fresh state: wall-clock rate, and the mean k_step time from HIP events
+    around 20 of every 500 iterations (the others replay graphs)."""
+    s = workloads_mod().make_sim(w)
+    s.step(50)
+    s.sync()
+    ms, nprof, done, steps = [0.0, 0.0, 0.0], 0, 0, 19950
+    t0 = time.perf_counter()
+    while done < steps:
+        k = min(20, steps - done)
+        s.profile_begin(k)
+        s.step(k)
+        m, n = s.profile_end()
+        ms = [a + b for a, b in zip(ms, m)]
+        nprof += n
+        done += k
+        rest = min(480, steps - done)
+        if rest > 0:
+            s.step(rest)
+            done += rest
+    s.sync()
+    wall = time.perf_counter() - t0
+    s.close()
+    kms = ms[1] / max(nprof, 1)
+    ach = B_ALG[precision] * cells / (kms * 1e-3) / 1e9
+    res = {"iterations": steps, "wall_s": round(wall, 3), "gcell_updates_per_s": round(cells * steps / wall / 1e9, 3),
+           "k_step_ms": round(kms, 4), "achieved_GBps": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
+           "profiled_iterations": nprof}
+    log("sustained 20,000-step run: %s" % res)
+    return res
+
+
+def workloads_mod():
+    import workloads
+    return workloads
 
 
 def ensemble_bench(n_grids, chunk=100):
@@ -102,10 +165,17 @@ def ensemble_bench(n_grids, chunk=100):
     it = iters[ids]
     cu = float((cells * it).sum())
     b.close()
+    gcu = cu / wall / 1e9
     res = {"workload": w["desc"], "scenarios": len(ids), "grids": n_grids,
            "cells_total": int(cells.sum()), "wall_s": round(wall, 3),
            "scenarios_per_s": round(len(ids) / wall, 3),
-           "gcell_updates_per_s": round(cu / wall / 1e9, 3),
+           "gcell_updates_per_s": round(gcu, 3),
+           # whole-run rate (wall clock, every launch) against HBM: the four
+           # scenarios of a grid share its bed and raster (26 B per
+           # scenario-cell when their tiles run together; 32 B otherwise)
+           "roofline": {"bound": "hbm", "bytes_per_cell": 26.0, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "achieved": round(26.0 * gcu, 1), "frac": round(26.0 * gcu / HBM_PEAK_GBS, 4),
+                        "frac_at_32_bytes": round(32.0 * gcu / HBM_PEAK_GBS, 4)},
            "iterations_min_max": [int(it.min()), int(it.max())], "batch_iterations": launches,
            "autotune_s": round(tune_s, 2), "segment_rows_by_size_bucket": rows}
     log("ensemble: %s" % res)
@@ -212,11 +282,13 @@ def accuracy_config2(size=1024):
     for k, s in runs.items():
         st = s.stats()
         d = np.abs(s.read(sw2d.F_H) - ha)
-        res[k] = {"max_dh_m": float(d.max()), "p99_9_dh_m": float(np.percentile(d, 99.9)),
+        mb = closure(st)
+        res[k] = {"gate_max_dh_le_1e-4_m": bool(d.max() <= 1e-4), "gate_mass_balance_le_1e-6": bool(mb <= 1e-6),
+                  "max_dh_m": float(d.max()), "p99_9_dh_m": float(np.percentile(d, 99.9)),
                   "p99_dh_m": float(np.percentile(d, 99)), "cells_dh_over_1e-4": int((d > 1e-4).sum()),
                   "cells_wet": int((ha > 1e-4).sum()),
                   "volume_rel_diff": abs(st["volume"] - sa["volume"]) / sa["volume"],
-                  "mass_balance": closure(st), "iterations": int(st["iterations"]),
+                  "mass_balance": mb, "iterations": int(st["iterations"]),
                   "terms_m3": {k2: round(st[k2], 4) for k2 in
                                ("volume", "rain_in", "outflow", "clamp_gain", "evap_out", "source_in")}}
         s.close()
@@ -359,6 +431,24 @@ def main():
                      "profiled_iterations": nprof},
         "sim_time_s": [round(st0["t"], 3), round(st1["t"], 3)],
     }
+
+    traffic = out["roofline"]["traffic"]
+    if traffic:
+        # rocprof-measured HBM bytes of one launch over the launch time
+        out["roofline"]["measured_traffic_GBps"] = round(traffic / (kern_ms * 1e-3) / 1e9, 1)
+        out["roofline"]["measured_bytes_per_cell"] = round(traffic / cells, 2)
+    # The configuration is a 20,000-step run from the release of the
+    # reservoir and the cost per step grows as rain wets the grid: a shorter
+    # timed window (the driver's) also gets the whole run's figures.
+    if args.steps + args.warmup < 20000:
+        s.close()
+        out["sustained_20000_steps"] = sustained_run(w, cells, args.precision)
+        s = workloads.make_sim(w)
+        s.step(args.warmup + args.steps)
+        s.sync()
+    else:
+        out["sustained_20000_steps"] = {"gcell_updates_per_s": out["value"], "k_step_ms": round(kern_ms, 4),
+                                        "frac": out["roofline"]["frac"], "same_as": "value"}
 
     h = U = V = None
     if not args.no_cpu or not args.no_fp64:

@@ -167,6 +167,10 @@ const char *sw2d_last_error(void);
 /* gpu_evaporation_calc L:561-582 over n meteorological records. */
 int sw2d_evaporation_table(double albedo, const double *T, const double *Rg,
                            int n, double dtrain, double *evapo_mm_out);
+/* x^1.3333330 for 0 < x < 1e300, the friction power of flux (L:195,
+ * L:312), as the FP64 step computes it on the device: a sequence of IEEE
+ * operations that gives the same bits on host and device (< 0.65 ulp). */
+double sw2d_friction_pow(double x);
 /* stream_flow / evaporation_load interpolation (L:533-538, L:547-552). */
 double sw2d_forcing_value(double time, const double *table, int n, double dT,
                           int mode);

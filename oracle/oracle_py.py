@@ -57,6 +57,8 @@ def lib():
         L.orc_get_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
         L.orc_stream_rate.restype = C.c_double
         L.orc_stream_rate.argtypes = [C.c_double, dp, C.c_int, C.c_double, C.c_int]
+        L.orc_pow_fric.restype = C.c_double
+        L.orc_pow_fric.argtypes = [C.c_double]
         L.orc_evap_table_entry.restype = C.c_double
         L.orc_evap_table_entry.argtypes = [C.c_double] * 4
         _lib = L
@@ -148,3 +150,8 @@ def stream_rate(time, table, dT, mode=0):
 
 def evap_entry(albedo, T, Rg, dtrain):
     return lib().orc_evap_table_entry(albedo, T, Rg, dtrain)
+
+
+def pow_fric(x):
+    """hf^1.3333330 as the oracle evaluates it (orc_pow_fric)."""
+    return lib().orc_pow_fric(float(x))

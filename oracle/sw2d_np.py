@@ -17,7 +17,6 @@ with h = 0 and bed 0.
 Each function vectorises one reference kernel over every id; np.where picks
 among branches that are all evaluated with the kernel's own operand order.
 """
-import math
 
 import numpy as np
 
@@ -25,9 +24,14 @@ G = 9.80          # L:809
 TH = 1.0e-4       # L:1130
 EXP = 1.3333330   # L:195, L:312
 
-# The C library pow (what the C oracle calls); numpy's own vectorised
-# power differs from it in the last bit for about 5% of arguments.
-_libm_pow = np.vectorize(math.pow, otypes=[np.float64])
+# The friction power hf^1.3333330: the C oracle's own IEEE-operation
+# sequence (orc_pow_fric), which the device FP64 step shares; it is pinned on
+# its own against a 60-digit evaluation (tests/test_oracle_pins.py), so this
+# transcription takes it from the oracle library rather than restating it.
+def _fric_pow(hface):
+    import oracle_py
+    f = np.vectorize(oracle_py.pow_fric, otypes=[np.float64])
+    return f(hface)
 
 
 def _interp(table, t, dT):
@@ -195,7 +199,7 @@ class RefLayoutSim:
         cross = np.where(cross_zero, 0.0, (dt2 / d_cross) * (mu - md))
         # friction
         sq = np.sqrt(fa * fa + fb * fb)
-        ex = _libm_pow(hface, EXP)
+        ex = _fric_pow(hface)
         ram = gg * self.rn * self.rn * sq / ex
         ram = np.where(hface <= TH, 0.00, ram)
         if along_first:

@@ -39,6 +39,77 @@
 
 #define FRICTION_EXP 1.3333330 /* L:195, L:312 (not 4/3) */
 
+/* The friction power hf^1.3333330 of L:195 and L:312.  The reference calls
+ * the CUDA math library's pow (not in the reference; CUDA 10.1, its ulp
+ * results unpinned).  The library pow functions of glibc and of the GPU
+ * device library differ from each other in the last bit for some arguments,
+ * and over a long run with wetting and drying such a bit grows through the
+ * switching dry-bed rules.  This oracle and the device FP64 step therefore
+ * both evaluate the power with the same sequence of IEEE operations (each
+ * correctly rounded or exact), written out here on its own: log x = e ln2 +
+ * 2 atanh(s) with s = (m - 1)/(m + 1), m in [1/sqrt2, sqrt2), and exp(t) =
+ * 2^k exp(r) with |r| <= ln2/2, both by their Taylor series, carried in
+ * double-double (Knuth two-sum, fma two-product) where a rounding would
+ * reach the result.  Error below 0.65 ulp (tests/test_oracle_pins.py checks
+ * it against a 60-digit decimal evaluation and against glibc pow). */
+double orc_pow_fric(double x) {
+  const double Y = FRICTION_EXP;
+  const double LN2_HI = 0x1.62e42fee00000p-1;  /* ln 2 rounded to 32 bits */
+  const double LN2_LO = 0x1.a39ef35793c76p-33; /* ln 2 - LN2_HI */
+  const double INV_LN2 = 0x1.71547652b82fep+0;
+  /* 1/(2j+1) for j = 12 .. 1 and 1/n! for n = 15 .. 2 */
+  static const double ATANH_C[12] = {
+      0x1.47ae147ae147bp-5, 0x1.642c8590b2164p-5, 0x1.8618618618618p-5, 0x1.af286bca1af28p-5,
+      0x1.e1e1e1e1e1e1ep-5, 0x1.1111111111111p-4, 0x1.3b13b13b13b14p-4, 0x1.745d1745d1746p-4,
+      0x1.c71c71c71c71cp-4, 0x1.2492492492492p-3, 0x1.999999999999ap-3, 0x1.5555555555555p-2};
+  static const double EXP_C[14] = {
+      0x1.ae7f3e733b81fp-41, 0x1.93974a8c07c9dp-37, 0x1.6124613a86d09p-33, 0x1.1eed8eff8d898p-29,
+      0x1.ae64567f544e4p-26, 0x1.27e4fb7789f5cp-22, 0x1.71de3a556c734p-19, 0x1.a01a01a01a01ap-16,
+      0x1.a01a01a01a01ap-13, 0x1.6c16c16c16c17p-10, 0x1.1111111111111p-7,  0x1.5555555555555p-5,
+      0x1.5555555555555p-3,  0x1.0000000000000p-1};
+  int e;
+  double m = frexp(x, &e); /* x = m 2^e, m in [1/2, 1) */
+  if (m < 0x1.6a09e667f3bcdp-1) {
+    m = m * 2.0;
+    e = e - 1;
+  }
+  double a = m - 1.0; /* exact */
+  double b = m + 1.0; /* b + bl = m + 1 exactly */
+  double bv = b - m;
+  double bl = (m - (b - bv)) + (1.0 - bv);
+  double s_hi = a / b;
+  double p = s_hi * b;
+  double pe = fma(s_hi, b, -p);
+  double s_lo = (((a - p) - pe) - s_hi * bl) / b;
+  double z = s_hi * s_hi;
+  double P = ATANH_C[0];
+  for (int j = 1; j < 12; j++) P = P * z + ATANH_C[j];
+  double L_hi = 2.0 * s_hi;
+  double L_lo = 2.0 * s_lo + 2.0 * (s_hi * z) * P;
+  double E_hi = (double)e * LN2_HI; /* exact */
+  double E_lo = (double)e * LN2_LO;
+  double S = E_hi + L_hi;
+  double Sv = S - E_hi;
+  double Se = (E_hi - (S - Sv)) + (L_hi - Sv);
+  double lo = Se + (E_lo + L_lo);
+  double lx_hi = S + lo;
+  double lx_lo = lo - (lx_hi - S);
+  double t_hi = Y * lx_hi;
+  double t_lo = fma(Y, lx_hi, -t_hi) + Y * lx_lo;
+  double k = rint(t_hi * INV_LN2);
+  double r_hi = t_hi - k * LN2_HI; /* exact */
+  double r_lo = t_lo - k * LN2_LO;
+  double r = r_hi + r_lo;
+  double r_e = r_lo - (r - r_hi);
+  double q = EXP_C[0];
+  for (int j = 1; j < 14; j++) q = q * r + EXP_C[j];
+  q = (r * r) * q;
+  double one_r = 1.0 + r;
+  double one_e = (1.0 - one_r) + r;
+  double res = one_r + (one_e + (r_e + q));
+  return ldexp(res, (int)k);
+}
+
 struct orc_sim {
   orc_params p;
   int R, C;
@@ -444,7 +515,7 @@ static void flux_stage(orc_sim *s, double D) {
       /* friction L:194-199 */
       double a = uu1[UI(i, j)], b = s->vva[UI(i, j)];
       double sqx = sqrt(a * a + b * b);
-      double expx = pow(hmv, FRICTION_EXP);
+      double expx = orc_pow_fric(hmv); /* pow(hmv, 1.3333330) */
       double rn = face_n(s, cB, cA);
       double ram = g * rn * rn * sqx / expx;
       if (hmv <= th) ram = 0.00;
@@ -503,7 +574,7 @@ static void flux_stage(orc_sim *s, double D) {
       /* friction L:311-316 */
       double a = s->uua[VI(i, j)], b = vv1[VI(i, j)];
       double sqy = sqrt(a * a + b * b);
-      double expy = pow(hnv, FRICTION_EXP);
+      double expy = orc_pow_fric(hnv); /* pow(hnv, 1.3333330) */
       double rn = face_n(s, cB, cA);
       double ram = g * rn * rn * sqy / expy;
       if (hnv <= th) ram = 0.00;

@@ -99,6 +99,8 @@ typedef struct {
 int orc_get_stats(const orc_sim *s, orc_stats *st);
 
 /* forcing functions restated from L:526-555 and L:589-601 */
+/* hf^1.3333330, the friction power of L:195 / L:312 (see the source) */
+double orc_pow_fric(double x);
 double orc_stream_rate(double time, const double *table, int n, double dT,
                        int mode);
 double orc_evap_table_entry(double albedo, double T, double Rg, double dtrain);

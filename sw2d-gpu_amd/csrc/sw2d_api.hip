@@ -130,6 +130,8 @@ size_t sw2d_footprint_bytes(int rows, int cols, int precision, int has_n) {
   return b + 65536;
 }
 
+double sw2d_friction_pow(double x) { return sw2d::pow_fric_f64(x); }
+
 double sw2d_forcing_value(double time, const double *table, int n, double dT, int mode) {
   return sw2d::forcing_value(time, table, n, dT, mode);
 }

@@ -145,6 +145,91 @@ __host__ __device__ inline double forcing_value(double time, const double *table
   return table[i0] + (table[i1] - table[i0]) / (dT * (time - dT * (it)));
 }
 
+// x^1.3333330 (the friction exponent of L:195, L:312) for 0 < x < 1e300,
+// from IEEE operations only: +, -, *, / and fma (each correctly rounded),
+// frexp, ldexp and rint (exact).  The same sequence therefore gives the same
+// bits on the device and on the host, wherever contraction is off (the FP64
+// kernels, the C ABI helper sw2d_friction_pow); the library pow functions of
+// the two sides differ in the last bit for some arguments.  Error below
+// 0.65 ulp: log x = e ln2 + 2 atanh(s), s = (m - 1)/(m + 1), m in [1/sqrt2,
+// sqrt2), and exp(t) = 2^k exp(r), |r| <= ln2/2, by series, each carried in
+// double-double where its rounding would matter (Dekker/Knuth two-sum and
+// fma two-product).
+__host__ __device__ inline double pow_fric_f64(double x) {
+  const double Y = 1.3333330;
+  const double LN2_HI = 0x1.62e42fee00000p-1;   // ln 2, 32 significant bits
+  const double LN2_LO = 0x1.a39ef35793c76p-33;  // ln 2 - LN2_HI
+  const double INV_LN2 = 0x1.71547652b82fep+0;
+  int e;
+  double m = frexp(x, &e);
+  if (m < 0x1.6a09e667f3bcdp-1) {  // 1/sqrt(2)
+    m = m * 2.0;
+    e = e - 1;
+  }
+  // s_hi + s_lo = (m - 1)/(m + 1); m - 1 is exact, b + bl = m + 1 exactly
+  const double a = m - 1.0;
+  const double b = m + 1.0;
+  const double bv = b - m;
+  const double bl = (m - (b - bv)) + (1.0 - bv);
+  const double s_hi = a / b;
+  const double p = s_hi * b;
+  const double pe = fma(s_hi, b, -p);
+  const double s_lo = (((a - p) - pe) - s_hi * bl) / b;
+  // log m = 2 s + 2 s^3 (1/3 + z/5 + ... + z^11/25), z = s^2 <= 0.0295
+  const double z = s_hi * s_hi;
+  double P = 0x1.47ae147ae147bp-5;  // 1/25
+  P = P * z + 0x1.642c8590b2164p-5;  // 1/23
+  P = P * z + 0x1.8618618618618p-5;  // 1/21
+  P = P * z + 0x1.af286bca1af28p-5;  // 1/19
+  P = P * z + 0x1.e1e1e1e1e1e1ep-5;  // 1/17
+  P = P * z + 0x1.1111111111111p-4;  // 1/15
+  P = P * z + 0x1.3b13b13b13b14p-4;  // 1/13
+  P = P * z + 0x1.745d1745d1746p-4;  // 1/11
+  P = P * z + 0x1.c71c71c71c71cp-4;  // 1/9
+  P = P * z + 0x1.2492492492492p-3;  // 1/7
+  P = P * z + 0x1.999999999999ap-3;  // 1/5
+  P = P * z + 0x1.5555555555555p-2;  // 1/3
+  const double L_hi = 2.0 * s_hi;
+  const double L_lo = 2.0 * s_lo + 2.0 * (s_hi * z) * P;
+  // log x = e ln2 + log m (e LN2_HI is exact)
+  const double E_hi = (double)e * LN2_HI;
+  const double E_lo = (double)e * LN2_LO;
+  const double S = E_hi + L_hi;
+  const double Sv = S - E_hi;
+  const double Se = (E_hi - (S - Sv)) + (L_hi - Sv);
+  const double lo = Se + (E_lo + L_lo);
+  const double lx_hi = S + lo;
+  const double lx_lo = lo - (lx_hi - S);
+  // t = Y log x
+  const double t_hi = Y * lx_hi;
+  const double t_lo = fma(Y, lx_hi, -t_hi) + Y * lx_lo;
+  // exp t = 2^k exp r; t_hi - k LN2_HI is exact
+  const double k = rint(t_hi * INV_LN2);
+  const double r_hi = t_hi - k * LN2_HI;
+  const double r_lo = t_lo - k * LN2_LO;
+  const double r = r_hi + r_lo;
+  const double r_e = r_lo - (r - r_hi);
+  double q = 0x1.ae7f3e733b81fp-41;  // 1/15!
+  q = q * r + 0x1.93974a8c07c9dp-37;  // 1/14!
+  q = q * r + 0x1.6124613a86d09p-33;  // 1/13!
+  q = q * r + 0x1.1eed8eff8d898p-29;  // 1/12!
+  q = q * r + 0x1.ae64567f544e4p-26;  // 1/11!
+  q = q * r + 0x1.27e4fb7789f5cp-22;  // 1/10!
+  q = q * r + 0x1.71de3a556c734p-19;  // 1/9!
+  q = q * r + 0x1.a01a01a01a01ap-16;  // 1/8!
+  q = q * r + 0x1.a01a01a01a01ap-13;  // 1/7!
+  q = q * r + 0x1.6c16c16c16c17p-10;  // 1/6!
+  q = q * r + 0x1.1111111111111p-7;   // 1/5!
+  q = q * r + 0x1.5555555555555p-5;   // 1/4!
+  q = q * r + 0x1.5555555555555p-3;   // 1/3!
+  q = q * r + 0x1.0000000000000p-1;   // 1/2!
+  q = (r * r) * q;
+  const double one_r = 1.0 + r;
+  const double one_e = (1.0 - one_r) + r;  // |r| < 1
+  const double res = one_r + (one_e + (r_e + q));
+  return ldexp(res, (int)k);
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
 // contiguous run of tiles so stencil halos shared by neighbouring tiles
 // stay in that XCD's L2.

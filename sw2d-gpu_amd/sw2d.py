@@ -25,7 +25,7 @@ EXPORTS = [
     "sw2d_record_dt", "sw2d_get_dt_history", "sw2d_step", "sw2d_sync",
     "sw2d_read", "sw2d_write_state", "sw2d_get_stats", "sw2d_snapshot",
     "sw2d_restore", "sw2d_footprint_bytes", "sw2d_last_error",
-    "sw2d_evaporation_table", "sw2d_forcing_value", "sw2d_case_load",
+    "sw2d_evaporation_table", "sw2d_forcing_value", "sw2d_friction_pow", "sw2d_case_load",
     "sw2d_case_free", "sw2d_case_get_info", "sw2d_case_get_array",
     "sw2d_case_source_cells", "sw2d_case_create", "sw2d_write_vtk",
     "sw2d_run_db", "sw2d_profile_begin", "sw2d_profile_end",
@@ -116,6 +116,8 @@ def lib():
         L.sw2d_footprint_bytes.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
         L.sw2d_last_error.restype = C.c_char_p
         L.sw2d_evaporation_table.argtypes = [C.c_double, dp, dp, C.c_int, C.c_double, dp]
+        L.sw2d_friction_pow.restype = C.c_double
+        L.sw2d_friction_pow.argtypes = [C.c_double]
         L.sw2d_forcing_value.restype = C.c_double
         L.sw2d_forcing_value.argtypes = [C.c_double, dp, C.c_int, C.c_double, C.c_int]
         L.sw2d_case_load.argtypes = [C.c_char_p, C.POINTER(vp)]
@@ -343,6 +345,11 @@ def run_db(db_root, **overrides):
 
 def footprint_bytes(rows, cols, precision=FP64, has_n=False):
     return lib().sw2d_footprint_bytes(rows, cols, precision, int(bool(has_n)))
+
+
+def friction_pow(x):
+    """x^1.3333330 as the FP64 step evaluates it (sw2d_friction_pow)."""
+    return lib().sw2d_friction_pow(float(x))
 
 
 def forcing_value(time, table, dT, mode=0):

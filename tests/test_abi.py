@@ -128,3 +128,14 @@ def test_footprint_is_about_32_bytes_per_cell_fp32():
     # the staging buffer is bounded: doubling the rows adds the arrays only
     b2 = sw2d.footprint_bytes(16384, 8192, sw2d.FP32, True)
     assert b2 - b < 34 * 8192 * 8192
+
+
+def test_library_friction_pow_is_the_oracle_sequence():
+    """sw2d_friction_pow (the sequence the FP64 step evaluates on the device)
+    gives the oracle's bits on every argument."""
+    import random
+    import oracle_py as O
+    rnd = random.Random(9)
+    for _ in range(20000):
+        x = 10 ** rnd.uniform(-4, 3)
+        assert sw2d.friction_pow(x) == O.pow_fric(x), x

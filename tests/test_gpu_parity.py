@@ -397,8 +397,14 @@ def test_gpu_fp32_mass_balance_under_uniform_rain():
     assert rel <= 1e-7
 
 
-def test_gpu_fp32_catchment_within_fp64_conditioning():
-    """Rain on a sloping noisy catchment with wetting and drying.  The dry-bed
+def test_gpu_fp32_catchment_conditioning_bound():
+    """A regression guard, not the accuracy gate: the pointwise gate of the
+    FP32 form (max |dh| <= 1e-4 m against FP64) is not met at wet/dry
+    switching cells (test_gpu_configs.py reports it on the 1024^2
+    catchment); this test bounds the FP32 deviation by the scheme's own
+    conditioning instead.
+
+    Rain on a sloping noisy catchment with wetting and drying.  The dry-bed
     rules switch discontinuously (a pit that fills to its spill level changes
     rule within one step), so any perturbation of the state moves single
     cells by ~1e-4 m: the FP64 scheme itself, run on the same bed moved by

@@ -502,3 +502,42 @@ def test_rain_block_map_on_still_water():
     w.step(150)
     for f in (O.H, O.U, O.V):
         assert np.array_equal(u.read(f), w.read(f))
+
+
+def _pow_args(n=6000, seed=5):
+    import random
+    rnd = random.Random(seed)
+    xs = [10 ** rnd.uniform(-4, 3) for _ in range(n)]
+    # the reduction boundaries, powers of two, the threshold, odd cases
+    xs += [1.0, 2.0, 0.5, 0.25, 1e-4, 1.0000000000000002, 0.9999999999999999,
+           0.70710678118654746, 0.70710678118654757, 1.4142135623730949, 1.4142135623730951, 3.0, 10.0,
+           100.0, 999.0]
+    return xs
+
+
+def test_friction_pow_error_against_60_digit_evaluation():
+    """The oracle's friction power (orc_pow_fric, shared with the device FP64
+    step) against x^1.3333330 evaluated with 60-digit decimal arithmetic:
+    within 0.65 ulp everywhere on [1e-4, 1e3] (the CUDA pow the reference
+    calls guarantees 2 ulp); a dropped series term or a wrong ln 2 split
+    gives errors of many ulps."""
+    from decimal import Decimal, getcontext
+    getcontext().prec = 60
+    Y = Decimal(1.3333330)
+    worst = 0.0
+    for x in _pow_args():
+        r = (Y * Decimal(x).ln()).exp()
+        err = abs(Decimal(O.pow_fric(x)) - r) / Decimal(math.ulp(float(r)))
+        worst = max(worst, float(err))
+    assert worst <= 0.65, worst
+    assert O.pow_fric(1.0) == 1.0 and O.pow_fric(2.0) == math.pow(2.0, 1.3333330)
+
+
+def test_friction_pow_agrees_with_glibc_within_one_ulp():
+    n_diff = 0
+    xs = _pow_args(seed=6)
+    for x in xs:
+        a, g = O.pow_fric(x), math.pow(x, 1.3333330)
+        assert abs(a - g) <= math.ulp(g), (x, a, g)
+        n_diff += a != g
+    assert n_diff <= 0.05 * len(xs)
