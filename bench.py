@@ -192,7 +192,10 @@ def serving_bench(n_requests, rate, chunk=20):
     w = workloads.config5(n_requests=n_requests, rate=rate)
     reqs = w["requests"]
     log("serving: %d requests generated in %.1f s" % (len(reqs), time.perf_counter() - t0))
-    budget = 250 * 10 ** 9   # admission budget, below the 288 GB of HBM
+    # admission by footprint against the MI355X's 288 GB, less what the
+    # device does not have free (runtime, other allocations) and 4 GiB
+    free, total = sw2d.device_memory()
+    budget = min(288 * 10 ** 9, free - (4 << 30))
     srv = sw2d.Server(sw2d.FP32, mem_budget=budget)
     p = dict(w["params"])
     ids = [None] * len(reqs)
@@ -229,7 +232,8 @@ def serving_bench(n_requests, rate, chunk=20):
                        "p99": round(float(np.percentile(queue, 99)), 3)},
            "service_s": {"p50": round(float(np.percentile(service, 50)), 3),
                          "p99": round(float(np.percentile(service, 99)), 3)},
-           "memory_budget_bytes": budget}
+           "memory_budget_bytes": budget, "device_memory_bytes": total,
+           "cell_updates_total": float((cells * it).sum())}
     log("serving: %s" % res)
     return res
 
@@ -315,7 +319,7 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the FP32-vs-FP64 accuracy leg (config 2)")
     ap.add_argument("--ensemble-grids", type=int, default=64)
-    ap.add_argument("--serving-requests", type=int, default=48)
+    ap.add_argument("--serving-requests", type=int, default=200)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy 5 times (the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")

@@ -156,6 +156,9 @@ int sw2d_get_stats(sw2d_ctx *ctx, sw2d_stats *out);
 int sw2d_snapshot(sw2d_ctx *ctx, void *buf, size_t *size);
 int sw2d_restore(sw2d_ctx *ctx, const void *buf, size_t size);
 
+/* Free and total memory of the current device (hipMemGetInfo). */
+int sw2d_device_memory(size_t *free_bytes, size_t *total_bytes);
+
 /* Device bytes a context with these dimensions and precision needs: its
  * arrays, the active-cell mask and the peak of the transient staging buffer
  * (8 B per cell) that uploads and read-backs pass through. */
@@ -218,6 +221,8 @@ int sw2d_batch_autotune(sw2d_batch *b, int iters);
 int sw2d_batch_tile_rows(sw2d_batch *b, int id, int *rows);
 int sw2d_batch_step(sw2d_batch *b, int64_t n_iter);
 int sw2d_batch_sync(sw2d_batch *b);
+/* *idle = 1 when the batch stream has no work left (does not wait). */
+int sw2d_batch_idle(sw2d_batch *b, int *idle);
 /* done[k] = 1 when scenario k (ids 0..n-1, removed ones report 1) has
  * reached its t_end; iters[k] = its committed iterations (either may be
  * NULL).  Synchronises.  Scenarios seen finished leave the work-list at
@@ -226,15 +231,20 @@ int sw2d_batch_status(sw2d_batch *b, int *done, int64_t *iters, int cap, int *n_
 
 /* ---- serving loop (no reference counterpart) -----------------------------
  * Continuous batching of scenario requests on one device.  Requests queue
- * on submit (the server keeps host copies of their inputs).  Each
- * sw2d_server_advance admits queued requests in arrival order while their
- * device footprint (sw2d_footprint_bytes) fits in the memory budget and a
- * slot is free, steps every admitted scenario n_iter iterations as one
- * batch (each scenario with its own clock and CFL step), and retires the
- * scenarios that reached their t_end: their final h, U, V and statistics
- * are copied to host memory and their device memory is freed for the next
- * admissions.  A scenario's result equals the same scenario run alone
- * (sw2d_create + sw2d_step), bitwise in h, U and V. */
+ * on submit (the server keeps host copies of their inputs) and are admitted
+ * in arrival order while their device footprint (sw2d_footprint_bytes) fits
+ * in the memory budget and a slot is free.  Admitted scenarios run in one
+ * batch per size class (cells < 2^18, < 2^21, larger), each batch on its own
+ * stream, so the classes advance concurrently and each at its own rate: a
+ * small scenario is not held to the launch rate of the large ones.  Each
+ * sw2d_server_advance waits until at least one class has finished its last
+ * chunk of iterations, and for every such class retires the scenarios that
+ * reached their t_end (final h, U, V and statistics go to host memory and
+ * their device memory returns to the budget), admits queued requests of
+ * that class, and enqueues its next chunk of iterations (n_iter, times 4
+ * for the smallest class and 2 for the middle one) without waiting for it.
+ * A scenario's result equals the same scenario run alone (sw2d_create +
+ * sw2d_step), bitwise in h, U and V. */
 typedef struct sw2d_server sw2d_server;
 #define SW2D_REQ_QUEUED 0
 #define SW2D_REQ_RUNNING 1
@@ -256,8 +266,9 @@ void sw2d_server_destroy(sw2d_server *s);
  * Fails with SW2D_E_MEMORY when the request alone exceeds the budget. */
 int sw2d_server_submit(sw2d_server *s, const sw2d_grid *grid, const sw2d_params *params,
                        const double *rain_mm, int n_rain, double dtrain, int *req_id);
-/* Admit, step n_iter iterations, retire (synchronises).  Optional outputs:
- * requests running and queued after the call. */
+/* Retire, admit and step the idle size classes (see above); waits only
+ * until one class is idle.  Optional outputs: requests running and queued
+ * after the call. */
 int sw2d_server_advance(sw2d_server *s, int64_t n_iter, int *n_running, int *n_queued);
 int sw2d_server_info(sw2d_server *s, int req_id, sw2d_request_info *info);
 /* Final field (SW2D_FIELD_H, _U or _V) or statistics of a retired request. */

@@ -132,6 +132,14 @@ size_t sw2d_footprint_bytes(int rows, int cols, int precision, int has_n) {
 
 double sw2d_friction_pow(double x) { return sw2d::pow_fric_f64(x); }
 
+int sw2d_device_memory(size_t *free_bytes, size_t *total_bytes) {
+  if (!free_bytes || !total_bytes) return fail(SW2D_E_INVALID, "null argument");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(SW2D_E_DEVICE, "no HIP device available");
+  HIPCHK(hipMemGetInfo(free_bytes, total_bytes));
+  return SW2D_OK;
+}
+
 double sw2d_forcing_value(double time, const double *table, int n, double dT, int mode) {
   return sw2d::forcing_value(time, table, n, dT, mode);
 }

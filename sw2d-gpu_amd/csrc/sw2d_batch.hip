@@ -299,6 +299,15 @@ int sw2d_batch_step(sw2d_batch *b, int64_t n_iter) {
   return SW2D_OK;
 }
 
+int sw2d_batch_idle(sw2d_batch *b, int *idle) {
+  if (!b || !idle) return fail(SW2D_E_INVALID, "null argument");
+  const hipError_t e = hipStreamQuery(b->stream);
+  if (e == hipSuccess) *idle = 1;
+  else if (e == hipErrorNotReady) *idle = 0;
+  else return fail(SW2D_E_DEVICE, std::string("batch stream: ") + hipGetErrorString(e));
+  return SW2D_OK;
+}
+
 int sw2d_batch_sync(sw2d_batch *b) {
   if (!b) return fail(SW2D_E_INVALID, "null batch");
   HIPCHK(hipStreamSynchronize(b->stream));

@@ -1,20 +1,26 @@
 // sw2d_server.cpp -- serving loop: continuous batching of flood-scenario
-// requests over one sw2d_batch.  Built only on the public C ABI (sw2d.h);
-// no reference counterpart (the reference runs one case directory at a
-// time, L:772-1486).
+// requests over one batch per size class.  Built only on the public C ABI
+// (sw2d.h); no reference counterpart (the reference runs one case directory
+// at a time, L:772-1486).
 //
-// Requests wait in a FIFO queue with host copies of their inputs.  Each
-// advance admits from the head of the queue while the admitted scenarios'
-// device footprint stays within the budget and a slot is free (strict
-// arrival order: a request that does not fit blocks those behind it), steps
-// the batch, and retires finished scenarios: their final fields and
-// statistics move to host memory and their device memory returns to the
-// budget.
+// Requests wait in a FIFO queue with host copies of their inputs.  Admission
+// takes the head of the queue while the admitted scenarios' device
+// footprint stays within the budget and a slot is free (strict arrival
+// order: a request that does not fit, or whose class is still running its
+// last chunk, blocks those behind it).  Each size class is a sw2d_batch with
+// its own stream: a chunk of iterations of one class is enqueued without
+// waiting, so the classes' kernels run concurrently and a class of small
+// grids advances at its own launch rate instead of the large grids'.  When a
+// class's stream is idle its finished scenarios are retired (their final
+// fields and statistics move to host memory, their device memory returns to
+// the budget), queued requests of that class are admitted and its next chunk
+// is enqueued.
 #include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <deque>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "sw2d.h"
@@ -40,9 +46,28 @@ struct Request {
   size_t footprint = 0;
   double submit_s = 0, admit_s = -1, finish_s = -1;
   int64_t iterations = 0;
-  int scen = -1;                 // batch scenario id while running
+  int cls = 0;                   // size class
+  int scen = -1;                 // scenario id in its class's batch while running
   std::vector<double> H, U, V;   // final fields once retired
   sw2d_stats stats{};
+};
+
+// Size classes by cells; a chunk of iterations is n_iter times the class's
+// multiplier (small grids take many cheap iterations per host round trip).
+constexpr int N_CLASS = 3;
+constexpr long long CLASS_CELLS[N_CLASS - 1] = {1LL << 18, 1LL << 21};
+constexpr int CLASS_MULT[N_CLASS] = {4, 2, 1};
+int class_of(int rows, int cols) {
+  const long long c = (long long)rows * cols;
+  int k = 0;
+  while (k < N_CLASS - 1 && c >= CLASS_CELLS[k]) k++;
+  return k;
+}
+
+struct Class {
+  sw2d_batch *batch = nullptr;
+  std::vector<int> running;  // request ids
+  int n_added = 0;           // scenario ids the batch has handed out (0 .. n_added-1)
 };
 
 }  // namespace
@@ -51,24 +76,29 @@ struct sw2d_server {
   int prec = SW2D_FP32;
   size_t budget = 0;
   int max_slots = 0;
-  sw2d_batch *batch = nullptr;
+  Class cls[N_CLASS];
   std::vector<Request> req;
   std::deque<int> queue;
-  std::vector<int> running;
   size_t used = 0;
-  int n_added = 0;  // scenario ids the batch has handed out (ids are 0 .. n_added-1)
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 
   double now() const {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
+  int n_running() const {
+    int n = 0;
+    for (const Class &c : cls) n += (int)c.running.size();
+    return n;
+  }
 
-  int admit() {
-    bool any = false;
+  // Admit from the head of the queue into idle classes.
+  int admit(const bool *idle, bool *touched) {
     while (!queue.empty()) {
       Request &r = req[queue.front()];
-      if (max_slots > 0 && (int)running.size() >= max_slots) break;
+      if (max_slots > 0 && n_running() >= max_slots) break;
       if (used + r.footprint > budget) break;
+      if (!idle[r.cls]) break;
+      Class &c = cls[r.cls];
       sw2d_grid g;
       g.rows = r.rows;
       g.cols = r.cols;
@@ -79,17 +109,17 @@ struct sw2d_server {
       g.h0 = r.h0.data();
       g.n_cell = r.has_n ? r.n.data() : nullptr;
       int id = -1;
-      int rc = sw2d_batch_add(batch, &g, &r.params, -1, &id);
+      int rc = sw2d_batch_add(c.batch, &g, &r.params, -1, &id);
       if (rc) return rc;
-      n_added = std::max(n_added, id + 1);
+      c.n_added = std::max(c.n_added, id + 1);
       sw2d_ctx *ctx = nullptr;
-      rc = sw2d_batch_scenario(batch, id, &ctx);
+      rc = sw2d_batch_scenario(c.batch, id, &ctx);
       if (!rc && !r.rain.empty())
         rc = sw2d_set_rain(ctx, r.rain.data(), (int)r.rain.size(), r.dtrain, nullptr, 0);
       if (rc) {
         // the scenario is in the batch but not admitted: take it out again
         // (the request stays at the head of the queue)
-        (void)sw2d_batch_remove(batch, id);
+        (void)sw2d_batch_remove(c.batch, id);
         return rc;
       }
       r.scen = id;
@@ -99,30 +129,32 @@ struct sw2d_server {
       std::vector<double>().swap(r.h0);
       std::vector<double>().swap(r.n);
       used += r.footprint;
-      running.push_back(queue.front());
+      c.running.push_back(queue.front());
       queue.pop_front();
-      any = true;
+      touched[r.cls] = true;
     }
-    return any ? sw2d_batch_touch(batch) : SW2D_OK;
+    return SW2D_OK;
   }
 
-  int retire() {
-    if (running.empty()) return SW2D_OK;
+  // Retire the finished scenarios of an idle class.
+  int retire(int k) {
+    Class &c = cls[k];
+    if (c.running.empty()) return SW2D_OK;
     int n = 0;
-    std::vector<int> done(std::max(n_added, 1));
-    std::vector<int64_t> iters(std::max(n_added, 1));
-    int rc = sw2d_batch_status(batch, done.data(), iters.data(), n_added, &n);
+    std::vector<int> done(std::max(c.n_added, 1));
+    std::vector<int64_t> iters(std::max(c.n_added, 1));
+    int rc = sw2d_batch_status(c.batch, done.data(), iters.data(), c.n_added, &n);
     if (rc) return rc;
     const double t = now();
     std::vector<int> still;
-    for (int k : running) {
-      Request &r = req[k];
+    for (int q : c.running) {
+      Request &r = req[q];
       if (!done[r.scen]) {
-        still.push_back(k);
+        still.push_back(q);
         continue;
       }
       sw2d_ctx *ctx = nullptr;
-      rc = sw2d_batch_scenario(batch, r.scen, &ctx);
+      rc = sw2d_batch_scenario(c.batch, r.scen, &ctx);
       if (rc) return rc;
       r.H.resize((size_t)r.rows * r.cols);
       r.U.resize((size_t)(r.rows + 1) * r.cols);
@@ -130,14 +162,49 @@ struct sw2d_server {
       if ((rc = sw2d_read(ctx, SW2D_FIELD_H, r.H.data())) || (rc = sw2d_read(ctx, SW2D_FIELD_U, r.U.data())) ||
           (rc = sw2d_read(ctx, SW2D_FIELD_V, r.V.data())) || (rc = sw2d_get_stats(ctx, &r.stats)))
         return rc;
-      if ((rc = sw2d_batch_remove(batch, r.scen))) return rc;
+      if ((rc = sw2d_batch_remove(c.batch, r.scen))) return rc;
       r.iterations = iters[r.scen];
       r.scen = -1;
       r.state = SW2D_REQ_DONE;
       r.finish_s = t;
       used -= r.footprint;
     }
-    running.swap(still);
+    c.running.swap(still);
+    return SW2D_OK;
+  }
+
+  int advance(int64_t n_iter) {
+    bool idle[N_CLASS], touched[N_CLASS] = {};
+    // wait until a class with running scenarios has finished its last chunk,
+    // or the request at the head of the queue can go into an idle class (or
+    // nothing runs at all)
+    for (;;) {
+      bool ready = false, busy = false;
+      for (int k = 0; k < N_CLASS; k++) {
+        int v = 1;
+        int rc = sw2d_batch_idle(cls[k].batch, &v);
+        if (rc) return rc;
+        idle[k] = v != 0;
+        if (!idle[k]) busy = true;
+        else if (!cls[k].running.empty()) ready = true;
+      }
+      if (!queue.empty() && idle[req[queue.front()].cls]) ready = true;
+      if (ready || !busy) break;
+      std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+    for (int k = 0; k < N_CLASS; k++)
+      if (idle[k]) {
+        int rc = retire(k);
+        if (rc) return rc;
+      }
+    int rc = admit(idle, touched);
+    if (rc) return rc;
+    for (int k = 0; k < N_CLASS; k++) {
+      if (!idle[k]) continue;
+      Class &c = cls[k];
+      if (touched[k] && (rc = sw2d_batch_touch(c.batch))) return rc;
+      if (!c.running.empty() && n_iter > 0 && (rc = sw2d_batch_step(c.batch, n_iter * CLASS_MULT[k]))) return rc;
+    }
     return SW2D_OK;
   }
 };
@@ -149,10 +216,12 @@ int sw2d_server_create(int precision, size_t mem_budget, int max_slots, sw2d_ser
   s->prec = precision;
   s->budget = mem_budget;
   s->max_slots = max_slots;
-  int rc = sw2d_batch_create(precision, &s->batch);
-  if (rc) {
-    delete s;
-    return rc;
+  for (Class &c : s->cls) {
+    int rc = sw2d_batch_create(precision, &c.batch);
+    if (rc) {
+      sw2d_server_destroy(s);
+      return rc;
+    }
   }
   *out = s;
   return SW2D_OK;
@@ -160,7 +229,8 @@ int sw2d_server_create(int precision, size_t mem_budget, int max_slots, sw2d_ser
 
 void sw2d_server_destroy(sw2d_server *s) {
   if (!s) return;
-  if (s->batch) sw2d_batch_destroy(s->batch);
+  for (Class &c : s->cls)
+    if (c.batch) sw2d_batch_destroy(c.batch);
   delete s;
 }
 
@@ -186,6 +256,7 @@ int sw2d_server_submit(sw2d_server *s, const sw2d_grid *grid, const sw2d_params 
   if (n_rain > 0) r.rain.assign(rain_mm, rain_mm + n_rain);
   r.dtrain = dtrain;
   r.footprint = sw2d_footprint_bytes(r.rows, r.cols, s->prec, r.has_n);
+  r.cls = class_of(r.rows, r.cols);
   if (r.footprint > s->budget) return srv_fail(SW2D_E_MEMORY, "request footprint exceeds the memory budget");
   r.submit_s = s->now();
   s->req.push_back(std::move(r));
@@ -196,10 +267,8 @@ int sw2d_server_submit(sw2d_server *s, const sw2d_grid *grid, const sw2d_params 
 
 int sw2d_server_advance(sw2d_server *s, int64_t n_iter, int *n_running, int *n_queued) {
   if (!s || n_iter < 0) return srv_fail(SW2D_E_INVALID, "advance arguments");
-  int rc = s->admit();
-  if (!rc && !s->running.empty() && n_iter > 0) rc = sw2d_batch_step(s->batch, n_iter);
-  if (!rc) rc = s->retire();
-  if (n_running) *n_running = (int)s->running.size();
+  const int rc = s->advance(n_iter);
+  if (n_running) *n_running = s->n_running();
   if (n_queued) *n_queued = (int)s->queue.size();
   return rc;
 }

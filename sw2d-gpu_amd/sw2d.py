@@ -33,7 +33,7 @@ EXPORTS = [
     "sw2d_gen_lognormal", "sw2d_gen_gamma", "sw2d_batch_create", "sw2d_batch_destroy",
     "sw2d_batch_add", "sw2d_batch_scenario", "sw2d_batch_touch", "sw2d_batch_remove",
     "sw2d_batch_set_tile_rows", "sw2d_batch_autotune", "sw2d_batch_tile_rows",
-    "sw2d_batch_step", "sw2d_batch_sync", "sw2d_batch_status",
+    "sw2d_batch_step", "sw2d_batch_sync", "sw2d_batch_status", "sw2d_batch_idle", "sw2d_device_memory",
     "sw2d_server_create", "sw2d_server_destroy", "sw2d_server_submit", "sw2d_server_advance",
     "sw2d_server_info", "sw2d_server_read", "sw2d_server_stats", "sw2d_server_release",
 ]
@@ -147,6 +147,8 @@ def lib():
         L.sw2d_batch_tile_rows.argtypes = [vp, C.c_int, ip]
         L.sw2d_batch_step.argtypes = [vp, C.c_int64]
         L.sw2d_batch_sync.argtypes = [vp]
+        L.sw2d_batch_idle.argtypes = [vp, C.POINTER(C.c_int)]
+        L.sw2d_device_memory.argtypes = [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
         L.sw2d_batch_status.argtypes = [vp, ip, C.POINTER(C.c_int64), C.c_int, ip]
         L.sw2d_server_create.argtypes = [C.c_int, C.c_size_t, C.c_int, C.POINTER(vp)]
         L.sw2d_server_destroy.argtypes = [vp]
@@ -347,6 +349,13 @@ def footprint_bytes(rows, cols, precision=FP64, has_n=False):
     return lib().sw2d_footprint_bytes(rows, cols, precision, int(bool(has_n)))
 
 
+def device_memory():
+    """(free, total) bytes of the current device."""
+    f, t = C.c_size_t(), C.c_size_t()
+    check(lib().sw2d_device_memory(C.byref(f), C.byref(t)))
+    return f.value, t.value
+
+
 def friction_pow(x):
     """x^1.3333330 as the FP64 step evaluates it (sw2d_friction_pow)."""
     return lib().sw2d_friction_pow(float(x))
@@ -460,6 +469,11 @@ class Batch:
 
     def sync(self):
         check(lib().sw2d_batch_sync(self.h_))
+
+    def idle(self):
+        v = C.c_int()
+        check(lib().sw2d_batch_idle(self.h_, C.byref(v)))
+        return bool(v.value)
 
     def status(self):
         n = len(self.shapes)
