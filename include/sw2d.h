@@ -136,6 +136,17 @@ int sw2d_sync(sw2d_ctx *ctx);
 #define SW2D_FIELD_Z 5  /* rows*cols bed as used on the device             */
 /* Copies one field to host memory in double precision (synchronises). */
 int sw2d_read(sw2d_ctx *ctx, int field, double *host_out);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops and harnesses that
+ * forbid hipMalloc).  Same as sw2d_read for SW2D_FIELD_H, _U, _V and _Z,
+ * but the conversion passes through a fixed 16 MiB staging array in device
+ * memory (one per GPU, held until the process ends) instead of a buffer
+ * allocated per call: it calls no hipMalloc, hipFree or allocating library
+ * routine.  Limit: rows of at most 2,097,152 values; the cell velocities
+ * (SW2D_FIELD_UU, _VV) are refused (use sw2d_read).  Calls on one GPU share
+ * the array: within a process they are serialised; calls that overlap on one
+ * GPU from several processes race.  Synchronises. */
+int sw2d_read_noalloc(sw2d_ctx *ctx, int field, double *host_out);
 /* Overwrites the state (h, U, V) from host arrays (synchronises). */
 int sw2d_write_state(sw2d_ctx *ctx, const double *h, const double *U,
                      const double *V);

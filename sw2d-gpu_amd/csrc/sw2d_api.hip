@@ -6,6 +6,9 @@
 // 9 launches + 9 device syncs + 4 H2D copies per iteration become ping-pong
 // state buffers, device-resident forcing tables and three stream-ordered
 // launches per iteration with no host synchronisation.
+#include <memory>
+#include <mutex>
+
 #include "sw2d_internal.hpp"
 
 namespace {
@@ -606,7 +609,12 @@ int sw2d_sync(sw2d_ctx *x) {
   return SW2D_OK;
 }
 
-int sw2d_read(sw2d_ctx *x, int field, double *out) {
+}  // extern "C"
+
+namespace {
+// One field into host memory.  stage: a device buffer of stage_n doubles
+// that the conversion passes through (nothing is allocated), or null.
+int read_field(sw2d_ctx *x, int field, double *out, double *stage, size_t stage_n) {
   if (!x || !out) return fail(SW2D_E_INVALID, "null argument");
   Ctrl c;
   int rc = x->fetch_ctrl(c);
@@ -614,13 +622,14 @@ int sw2d_read(sw2d_ctx *x, int field, double *out) {
   const int p = c.parity, R = x->R, C = x->C;
   switch (field) {
     case SW2D_FIELD_H:  // inactive cells (-1) read as 0
-      return x->read_pitched(x->h[p], R, C, out, x->RS, true);
+      return x->read_pitched(x->h[p], R, C, out, x->RS, true, stage, stage_n);
     case SW2D_FIELD_U:
-      return x->read_pitched(x->U[p], R + 1, C, out, x->RS);
+      return x->read_pitched(x->U[p], R + 1, C, out, x->RS, false, stage, stage_n);
     case SW2D_FIELD_V:
-      return x->read_pitched(x->V[p], R, C + 1, out, x->RS);
+      return x->read_pitched(x->V[p], R, C + 1, out, x->RS, false, stage, stage_n);
     case SW2D_FIELD_UU:
     case SW2D_FIELD_VV: {
+      if (stage) return fail(SW2D_E_INVALID, "cell velocities are read with sw2d_read");
       if (!x->d_uu) {
         rc = x->alloc((void **)&x->d_uu, (size_t)R * C * sizeof(double));
         rc = rc ? rc : x->alloc((void **)&x->d_vv, (size_t)R * C * sizeof(double));
@@ -633,7 +642,7 @@ int sw2d_read(sw2d_ctx *x, int field, double *out) {
       return SW2D_OK;
     }
     case SW2D_FIELD_Z:
-      rc = x->read_pitched(x->z, R, C, out, x->RSS);
+      rc = x->read_pitched(x->z, R, C, out, x->RSS, false, stage, stage_n);
       if (!rc && x->prec == SW2D_FP32)
         for (int i = 0; i < R; i++)
           for (int j = 0; j < C; j++)
@@ -646,6 +655,68 @@ int sw2d_read(sw2d_ctx *x, int field, double *out) {
     default:
       return fail(SW2D_E_INVALID, "unknown field");
   }
+}
+
+constexpr size_t NOALLOC_STAGE_DOUBLES = size_t(2) << 20;  // 16 MiB
+}  // namespace
+
+// The staging buffer of sw2d_read_noalloc: a fixed array in device memory,
+// one copy per GPU for the life of the process.  Host code never uses the
+// array's name as an address (on the host it only names the symbol): the
+// device address comes from hipGetSymbolAddress, once per device.
+static __device__ __align__(256) double scratch[NOALLOC_STAGE_DOUBLES];
+
+namespace {
+struct ScratchTable {
+  std::mutex m;
+  std::vector<double *> ptr;   // per device, resolved on first use
+  std::vector<std::unique_ptr<std::mutex>> use;  // serialises calls per device
+};
+ScratchTable &scratch_table() {
+  static ScratchTable t;
+  return t;
+}
+// device address of the current device's scratch array and its lock
+int scratch_for_current_device(double **p, std::mutex **lock) {
+  int dev = 0, ndev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipGetDeviceCount(&ndev));
+  ScratchTable &t = scratch_table();
+  std::lock_guard<std::mutex> g(t.m);
+  if ((int)t.ptr.size() < ndev) {
+    t.ptr.resize(ndev, nullptr);
+    while ((int)t.use.size() < ndev) t.use.emplace_back(new std::mutex());
+  }
+  if (!t.ptr[dev]) {
+    void *a = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&a, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !a)
+      return fail(SW2D_E_DEVICE, std::string("staging symbol address: ") + hipGetErrorString(e));
+    t.ptr[dev] = (double *)a;
+  }
+  *p = t.ptr[dev];
+  *lock = t.use[dev].get();
+  return SW2D_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int sw2d_read(sw2d_ctx *x, int field, double *out) { return read_field(x, field, out, nullptr, 0); }
+
+int sw2d_read_noalloc(sw2d_ctx *x, int field, double *out) {
+  if (!x || !out) return fail(SW2D_E_INVALID, "null argument");
+  if (field == SW2D_FIELD_UU || field == SW2D_FIELD_VV)
+    return fail(SW2D_E_INVALID, "sw2d_read_noalloc reads h, U, V and z (cell velocities: sw2d_read)");
+  const int cols = field == SW2D_FIELD_V ? x->C + 1 : x->C;
+  if ((size_t)cols > NOALLOC_STAGE_DOUBLES)
+    return fail(SW2D_E_INVALID, "sw2d_read_noalloc: a row of more than 2,097,152 values");
+  double *p = nullptr;
+  std::mutex *lock = nullptr;
+  int rc = scratch_for_current_device(&p, &lock);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> g(*lock);
+  return read_field(x, field, out, p, NOALLOC_STAGE_DOUBLES);
 }
 
 int sw2d_write_state(sw2d_ctx *x, const double *h, const double *U, const double *V) {

@@ -213,12 +213,20 @@ struct sw2d_ctx {
   // double array: the conversion runs on the device (clamp0: negative values
   // read as 0), a chunk of rows at a time through a stream-ordered staging
   // buffer of at most sw2d::STAGING_MAX bytes.
-  int read_pitched(const void *src, int rows, int cols, double *out, int pitch, bool clamp0 = false) {
+  // With `stage` (a device buffer of stage_n doubles) the rows pass through
+  // it instead, and nothing is allocated.
+  int read_pitched(const void *src, int rows, int cols, double *out, int pitch, bool clamp0 = false,
+                   double *stage = nullptr, size_t stage_n = 0) {
     const size_t n = (size_t)rows * cols;
     if (n == 0) return SW2D_OK;
-    const int cr = sw2d::staging_rows(rows, cols);
-    double *d_out = nullptr;
-    HIPCHK(hipMallocAsync((void **)&d_out, (size_t)cr * cols * sizeof(double), stream));
+    int cr = sw2d::staging_rows(rows, cols);
+    double *d_out = stage;
+    if (stage) {
+      if (stage_n < (size_t)cols) return fail(SW2D_E_INVALID, "a row is wider than the staging buffer");
+      cr = (int)std::min<size_t>((size_t)rows, stage_n / (size_t)cols);
+    } else {
+      HIPCHK(hipMallocAsync((void **)&d_out, (size_t)cr * cols * sizeof(double), stream));
+    }
     hipError_t e = hipSuccess;
     for (int a = 0; a < rows && e == hipSuccess; a += cr) {
       const int nr = std::min(cr, rows - a);
@@ -228,7 +236,7 @@ struct sw2d_ctx {
         e = hipMemcpyAsync(out + (size_t)a * cols, d_out, (size_t)nr * cols * sizeof(double),
                            hipMemcpyDeviceToHost, stream);
     }
-    const hipError_t f = hipFreeAsync(d_out, stream);
+    const hipError_t f = stage ? hipSuccess : hipFreeAsync(d_out, stream);
     const hipError_t y = hipStreamSynchronize(stream);
     if (e == hipSuccess) e = f != hipSuccess ? f : y;
     if (e != hipSuccess) return fail(SW2D_E_DEVICE, std::string("read: ") + hipGetErrorString(e));

@@ -23,7 +23,7 @@ EXPORTS = [
     "sw2d_params_default", "sw2d_create", "sw2d_destroy", "sw2d_set_rain",
     "sw2d_set_sources", "sw2d_set_evaporation", "sw2d_set_dt_schedule",
     "sw2d_record_dt", "sw2d_get_dt_history", "sw2d_step", "sw2d_sync",
-    "sw2d_read", "sw2d_write_state", "sw2d_get_stats", "sw2d_snapshot",
+    "sw2d_read", "sw2d_read_noalloc", "sw2d_write_state", "sw2d_get_stats", "sw2d_snapshot",
     "sw2d_restore", "sw2d_footprint_bytes", "sw2d_last_error",
     "sw2d_evaporation_table", "sw2d_forcing_value", "sw2d_friction_pow", "sw2d_case_load",
     "sw2d_case_free", "sw2d_case_get_info", "sw2d_case_get_array",
@@ -108,6 +108,7 @@ def lib():
         L.sw2d_step.argtypes = [vp, C.c_int64]
         L.sw2d_sync.argtypes = [vp]
         L.sw2d_read.argtypes = [vp, C.c_int, dp]
+        L.sw2d_read_noalloc.argtypes = [vp, C.c_int, dp]
         L.sw2d_write_state.argtypes = [vp, dp, dp, dp]
         L.sw2d_get_stats.argtypes = [vp, C.POINTER(Stats)]
         L.sw2d_snapshot.argtypes = [vp, vp, C.POINTER(C.c_size_t)]
@@ -275,6 +276,16 @@ class Sim:
                  F_U: (R + 1, Cc), F_V: (R, Cc + 1)}[field]
         out = np.empty(shape)
         check(lib().sw2d_read(self.h_, field, _dp(out)))
+        return out
+
+    def read_noalloc(self, field):
+        """sw2d_read_noalloc: h, U, V or z through the library's fixed device
+        staging array (no device allocation)."""
+        R, Cc = self.rows, self.cols
+        shape = {F_H: (R, Cc), F_UU: (R, Cc), F_VV: (R, Cc), F_Z: (R, Cc),
+                 F_U: (R + 1, Cc), F_V: (R, Cc + 1)}[field]
+        out = np.empty(shape)
+        check(lib().sw2d_read_noalloc(self.h_, field, _dp(out)))
         return out
 
     def write_state(self, h, U, V):
