@@ -147,6 +147,18 @@ int sw2d_read(sw2d_ctx *ctx, int field, double *host_out);
  * the array: within a process they are serialised; calls that overlap on one
  * GPU from several processes race.  Synchronises. */
 int sw2d_read_noalloc(sw2d_ctx *ctx, int field, double *host_out);
+/* Output snapshots that overlap the copy to host memory with the next
+ * iterations (the reference synchronises, copies h, uu, vv and writes its
+ * files while the GPU idles, L:1339-1472).  sw2d_output_begin enqueues, on
+ * the context stream after the work queued so far, the depth (inactive
+ * cells 0) and the cell velocities uu_vv (L:459-484) of the state reached
+ * into a device snapshot, whose copy into one of two pinned host slots runs
+ * on a side stream; it returns at once with the slot.  sw2d_output_wait
+ * waits for that copy and returns rows*cols arrays in the slot (valid until
+ * sw2d_output_begin hands out the slot again, two calls later).  Not for
+ * batch scenarios. */
+int sw2d_output_begin(sw2d_ctx *ctx, int *slot);
+int sw2d_output_wait(sw2d_ctx *ctx, int slot, const double **h, const double **vv, const double **uu);
 /* Overwrites the state (h, U, V) from host arrays (synchronises). */
 int sw2d_write_state(sw2d_ctx *ctx, const double *h, const double *U,
                      const double *V);

@@ -635,7 +635,8 @@ int read_field(sw2d_ctx *x, int field, double *out, double *stage, size_t stage_
         rc = rc ? rc : x->alloc((void **)&x->d_vv, (size_t)R * C * sizeof(double));
         if (rc) return rc;
       }
-      HIPCHK(sw2d::launch_cell_velocity(x->prec, x->d_grid, 0, x->d_uu, x->d_vv, x->vol_blocks, x->stream));
+      HIPCHK(sw2d::launch_cell_velocity(x->prec, x->d_grid, 0, x->d_uu, x->d_vv, nullptr, x->vol_blocks,
+                                        x->stream));
       HIPCHK(hipStreamSynchronize(x->stream));
       HIPCHK(hipMemcpy(out, field == SW2D_FIELD_UU ? x->d_uu : x->d_vv, (size_t)R * C * sizeof(double),
                        hipMemcpyDeviceToHost));
@@ -717,6 +718,45 @@ int sw2d_read_noalloc(sw2d_ctx *x, int field, double *out) {
   if (rc) return rc;
   std::lock_guard<std::mutex> g(*lock);
   return read_field(x, field, out, p, NOALLOC_STAGE_DOUBLES);
+}
+
+int sw2d_output_begin(sw2d_ctx *x, int *slot) {
+  if (!x || !slot) return fail(SW2D_E_INVALID, "null argument");
+  if (x->owner) return fail(SW2D_E_STATE, "a batch scenario is read with sw2d_read");
+  const size_t n = (size_t)x->R * x->C;
+  if (!x->d_snap) {
+    int rc = x->alloc((void **)&x->d_snap, 3 * n * sizeof(double));
+    if (rc) return rc;
+    HIPCHK(hipStreamCreateWithFlags(&x->side, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&x->snap_ready, hipEventDisableTiming));
+    for (int k = 0; k < 2; k++) {
+      HIPCHK(hipEventCreateWithFlags(&x->snap_done[k], hipEventDisableTiming));
+      HIPCHK(hipHostMalloc((void **)&x->h_snap[k], 3 * n * sizeof(double), hipHostMallocDefault));
+    }
+  }
+  const int k = x->snap_next;
+  // the device snapshot is rewritten only after the other slot's copy read it
+  if (x->snap_used[k ^ 1]) HIPCHK(hipStreamWaitEvent(x->stream, x->snap_done[k ^ 1], 0));
+  HIPCHK(sw2d::launch_cell_velocity(x->prec, x->d_grid, 0, x->d_snap + 2 * n, x->d_snap + n, x->d_snap,
+                                    x->vol_blocks, x->stream));
+  HIPCHK(hipEventRecord(x->snap_ready, x->stream));
+  HIPCHK(hipStreamWaitEvent(x->side, x->snap_ready, 0));
+  HIPCHK(hipMemcpyAsync(x->h_snap[k], x->d_snap, 3 * n * sizeof(double), hipMemcpyDeviceToHost, x->side));
+  HIPCHK(hipEventRecord(x->snap_done[k], x->side));
+  x->snap_used[k] = true;
+  x->snap_next = k ^ 1;
+  *slot = k;
+  return SW2D_OK;
+}
+
+int sw2d_output_wait(sw2d_ctx *x, int slot, const double **h, const double **vv, const double **uu) {
+  if (!x || slot < 0 || slot > 1 || !x->snap_used[slot]) return fail(SW2D_E_INVALID, "output slot");
+  HIPCHK(hipEventSynchronize(x->snap_done[slot]));
+  const size_t n = (size_t)x->R * x->C;
+  if (h) *h = x->h_snap[slot];
+  if (vv) *vv = x->h_snap[slot] + n;
+  if (uu) *uu = x->h_snap[slot] + 2 * n;
+  return SW2D_OK;
 }
 
 int sw2d_write_state(sw2d_ctx *x, const double *h, const double *U, const double *V) {

@@ -26,8 +26,8 @@ hipError_t launch_step_only_f32(const IterLaunch &a, hipStream_t s) {
 }
 
 hipError_t launch_cfl_state_f64(const void *grids, int gid, int blocks, hipStream_t s);
-hipError_t launch_cell_velocity_f64(const void *grids, int gid, double *uu, double *vv, int blocks,
-                                    hipStream_t s);
+hipError_t launch_cell_velocity_f64(const void *grids, int gid, double *uu, double *vv, double *hout,
+                                    int blocks, hipStream_t s);
 hipError_t launch_volume_f64(const void *grids, int gid, double *partial, int blocks, hipStream_t s);
 
 hipError_t launch_cfl_state(int precision, const void *grids, int gid, int blocks, hipStream_t s) {
@@ -37,10 +37,10 @@ hipError_t launch_cfl_state(int precision, const void *grids, int gid, int block
   return hipGetLastError();
 }
 hipError_t launch_cell_velocity(int precision, const void *grids, int gid, double *uu, double *vv,
-                                int blocks, hipStream_t s) {
-  if (precision == 0) return launch_cell_velocity_f64(grids, gid, uu, vv, blocks, s);
+                                double *hout, int blocks, hipStream_t s) {
+  if (precision == 0) return launch_cell_velocity_f64(grids, gid, uu, vv, hout, blocks, s);
   hipLaunchKernelGGL(k_cell_velocity<float>, dim3(blocks), dim3(BLOCK), 0, s,
-                     static_cast<const Grid<float> *>(grids), gid, uu, vv);
+                     static_cast<const Grid<float> *>(grids), gid, uu, vv, hout);
   return hipGetLastError();
 }
 hipError_t launch_volume(int precision, const void *grids, int gid, double *partial, int blocks,

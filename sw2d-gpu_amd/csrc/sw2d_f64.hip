@@ -40,10 +40,10 @@ hipError_t launch_cfl_state_f64(const void *grids, int gid, int blocks, hipStrea
                      static_cast<const Grid<double> *>(grids), gid);
   return hipGetLastError();
 }
-hipError_t launch_cell_velocity_f64(const void *grids, int gid, double *uu, double *vv, int blocks,
-                                    hipStream_t s) {
+hipError_t launch_cell_velocity_f64(const void *grids, int gid, double *uu, double *vv, double *hout,
+                                    int blocks, hipStream_t s) {
   hipLaunchKernelGGL(k_cell_velocity<double>, dim3(blocks), dim3(BLOCK), 0, s,
-                     static_cast<const Grid<double> *>(grids), gid, uu, vv);
+                     static_cast<const Grid<double> *>(grids), gid, uu, vv, hout);
   return hipGetLastError();
 }
 hipError_t launch_volume_f64(const void *grids, int gid, double *partial, int blocks, hipStream_t s) {

@@ -140,8 +140,26 @@ struct sw2d_ctx {
 
   GraphCache graph;
 
+  // output snapshots (sw2d_output_begin / sw2d_output_wait): the device
+  // snapshot of (h, vv, uu), its two pinned host copies and the side stream
+  // that copies it while the context stream goes on stepping
+  double *d_snap = nullptr;
+  double *h_snap[2] = {};
+  hipEvent_t snap_done[2] = {};
+  hipEvent_t snap_ready = nullptr;
+  hipStream_t side = nullptr;
+  int snap_next = 0;
+  bool snap_used[2] = {};
+
   ~sw2d_ctx() {
     if (stream) (void)hipStreamSynchronize(stream);
+    if (side) (void)hipStreamSynchronize(side);
+    for (int k = 0; k < 2; k++) {
+      if (snap_done[k]) (void)hipEventDestroy(snap_done[k]);
+      if (h_snap[k]) (void)hipHostFree(h_snap[k]);
+    }
+    if (snap_ready) (void)hipEventDestroy(snap_ready);
+    if (side) (void)hipStreamDestroy(side);
     graph.reset();
     for (hipEvent_t e : prof_ev) (void)hipEventDestroy(e);
     for (void *p : allocs) (void)hipFree(base_of(p));

@@ -410,7 +410,6 @@ int sw2d_run_db(const char *db_root, const sw2d_params *overrides) {
       gk[g] = outy * I.cols - (I.cols - (outx + 1));
     }
     const size_t ncell = (size_t)I.rows * I.cols;
-    std::vector<double> h(ncell), uu(ncell), vv(ncell);
     const bool adaptive = overrides && overrides->cfl > 0;
     double time = I.time0;
     long long mstep = 0;
@@ -418,33 +417,54 @@ int sw2d_run_db(const char *db_root, const sw2d_params *overrides) {
     std::string tempo;
     long long pending = 0;
     auto t_start = std::chrono::steady_clock::now();
-    auto write_output = [&]() -> int {
+    // An output is a snapshot of the state on the device (sw2d_output_begin)
+    // whose copy to host memory runs beside the next iterations; its files
+    // are written once the iterations toward the next output are queued, so
+    // the host formats output k while the device computes toward k + 1.
+    struct Out {
+      int slot;
+      std::string tempo;
+      bool first;  // out0 == 0: depth from the initial condition (L:1415-1416)
+    };
+    std::vector<Out> outs;  // at most one waiting to be written
+    auto write_files = [&](const Out &o) -> int {
+      const double *h = nullptr, *vv = nullptr, *uu = nullptr;
+      int r = sw2d_output_wait(x, o.slot, &h, &vv, &uu);
+      if (r) return r;
+      const double *depth = o.first ? c->ic.data() : h;
+      // gauges, raster order (L:1380-1412)
+      if (I.out_depth == 1 && I.out_outlet_on == 1) {
+        int npout = 0;
+        for (size_t km = 0; km < ncell; km++)
+          for (int g = 0; g < I.n_out; g++)
+            if ((long)km == gk[g]) {
+              npout++;
+              std::fprintf(wl, npout == I.n_out ? " %lf\n" : " %lf", depth[km]);
+            }
+      }
+      const std::string path = outdir + "Results_" + o.tempo + ".vtk";
+      return sw2d_write_vtk(path.c_str(), c, depth, vv, uu, c->dem.data());
+    };
+    auto flush_older = [&]() -> int {  // write every output but the newest
+      int r = SW2D_OK;
+      while (!r && outs.size() > 1) {
+        r = write_files(outs.front());
+        outs.erase(outs.begin());
+      }
+      return r;
+    };
+    auto take_output = [&]() -> int {
       int r = sw2d_step(x, pending);
       pending = 0;
-      if (!r) r = sw2d_sync(x);
+      int slot = 0;
+      if (!r) r = sw2d_output_begin(x, &slot);
       if (r) return r;
       std::ostringstream os;
       if (out0 > 0) os << std::round(mstep * I.dt / I.dkout);  // L:1343
       else os << 0;
       tempo = os.str();
-      if (I.out_depth == 1 && out0 != 0) r = sw2d_read(x, SW2D_FIELD_H, h.data());
-      if (!r && I.out_velocity_x == 1) r = sw2d_read(x, SW2D_FIELD_VV, vv.data());
-      if (!r && I.out_velocity_y == 1) r = sw2d_read(x, SW2D_FIELD_UU, uu.data());
-      if (r) return r;
-      // gauges, raster order (L:1380-1412)
-      if (I.out_depth == 1 && I.out_outlet_on == 1) {
-        int npout = 0;
-        const double *src = out0 == 0 ? c->ic.data() : h.data();
-        for (size_t km = 0; km < ncell; km++)
-          for (int g = 0; g < I.n_out; g++)
-            if ((long)km == gk[g]) {
-              npout++;
-              std::fprintf(wl, npout == I.n_out ? " %lf\n" : " %lf", src[km]);
-            }
-      }
-      const std::string path = outdir + "Results_" + tempo + ".vtk";
-      return sw2d_write_vtk(path.c_str(), c, out0 == 0 ? c->ic.data() : h.data(), vv.data(), uu.data(),
-                            c->dem.data());
+      outs.push_back(Out{slot, tempo, out0 == 0});
+      return SW2D_OK;
     };
     if (!adaptive) {
       // the host mirrors the device clock exactly (same FP64 additions)
@@ -456,12 +476,12 @@ int sw2d_run_db(const char *db_root, const sw2d_params *overrides) {
         time = time + I.dt;
         mstep += 2;
         if ((mstep % lkout == 0) || (out0 == 0)) {
-          rc = write_output();
+          rc = take_output();      // queues the iterations up to here and the snapshot
+          if (!rc) rc = flush_older();
           if (rc) break;
         }
       }
       if (!rc && pending) rc = sw2d_step(x, pending);
-      if (!rc) rc = sw2d_sync(x);
     } else {
       // adaptive step: the device owns the clock and the end test; the host
       // advances to the next iteration whose mstep is a multiple of lkout
@@ -474,6 +494,7 @@ int sw2d_run_db(const char *db_root, const sw2d_params *overrides) {
         pending = target - mstep / 2;
         rc = sw2d_step(x, pending);
         pending = 0;
+        if (!rc) rc = flush_older();  // the previous output, while the device steps
         if (!rc) rc = sw2d_get_stats(x, &st);
         if (rc) break;
         const long long did = st.iterations - mstep / 2;
@@ -481,11 +502,15 @@ int sw2d_run_db(const char *db_root, const sw2d_params *overrides) {
         out0 += did;
         mstep = 2 * st.iterations;
         std::printf(" %d\n", int(st.t));
-        if ((mstep % lkout == 0) || first) rc = write_output();
+        if ((mstep % lkout == 0) || first) rc = take_output();  // written after the next steps are queued
         first = false;
         if (st.done) break;
       }
     }
+    for (const Out &o : outs)
+      if (!rc) rc = write_files(o);
+    outs.clear();
+    if (!rc) rc = sw2d_sync(x);
     const double duration = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
     std::cout << "printf: " << duration << '\n';
     FILE *ts = std::fopen((outdir + "TimeSimu_" + tempo + ".txt").c_str(), "w");

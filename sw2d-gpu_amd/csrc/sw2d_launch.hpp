@@ -90,8 +90,9 @@ hipError_t launch_iteration_f32(const IterLaunch &a, hipStream_t s);
 hipError_t launch_step_only(int precision, const IterLaunch &a, hipStream_t s);
 hipError_t launch_step_only_f32(const IterLaunch &a, hipStream_t s);
 hipError_t launch_cfl_state(int precision, const void *grids, int gid, int blocks, hipStream_t s);
+// cell velocities uu, vv (uu_vv L:472-479) and, with hout, the depth
 hipError_t launch_cell_velocity(int precision, const void *grids, int gid, double *uu, double *vv,
-                                int blocks, hipStream_t s);
+                                double *hout, int blocks, hipStream_t s);
 hipError_t launch_volume(int precision, const void *grids, int gid, double *partial, int blocks,
                          hipStream_t s);
 

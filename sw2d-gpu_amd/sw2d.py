@@ -23,7 +23,7 @@ EXPORTS = [
     "sw2d_params_default", "sw2d_create", "sw2d_destroy", "sw2d_set_rain",
     "sw2d_set_sources", "sw2d_set_evaporation", "sw2d_set_dt_schedule",
     "sw2d_record_dt", "sw2d_get_dt_history", "sw2d_step", "sw2d_sync",
-    "sw2d_read", "sw2d_read_noalloc", "sw2d_write_state", "sw2d_get_stats", "sw2d_snapshot",
+    "sw2d_read", "sw2d_read_noalloc", "sw2d_output_begin", "sw2d_output_wait", "sw2d_write_state", "sw2d_get_stats", "sw2d_snapshot",
     "sw2d_restore", "sw2d_footprint_bytes", "sw2d_last_error",
     "sw2d_evaporation_table", "sw2d_forcing_value", "sw2d_friction_pow", "sw2d_case_load",
     "sw2d_case_free", "sw2d_case_get_info", "sw2d_case_get_array",
@@ -109,6 +109,8 @@ def lib():
         L.sw2d_sync.argtypes = [vp]
         L.sw2d_read.argtypes = [vp, C.c_int, dp]
         L.sw2d_read_noalloc.argtypes = [vp, C.c_int, dp]
+        L.sw2d_output_begin.argtypes = [vp, C.POINTER(C.c_int)]
+        L.sw2d_output_wait.argtypes = [vp, C.c_int, C.POINTER(dp), C.POINTER(dp), C.POINTER(dp)]
         L.sw2d_write_state.argtypes = [vp, dp, dp, dp]
         L.sw2d_get_stats.argtypes = [vp, C.POINTER(Stats)]
         L.sw2d_snapshot.argtypes = [vp, vp, C.POINTER(C.c_size_t)]
@@ -277,6 +279,19 @@ class Sim:
         out = np.empty(shape)
         check(lib().sw2d_read(self.h_, field, _dp(out)))
         return out
+
+    def output_begin(self):
+        """sw2d_output_begin: snapshot (h, vv, uu) asynchronously; the slot."""
+        k = C.c_int()
+        check(lib().sw2d_output_begin(self.h_, C.byref(k)))
+        return k.value
+
+    def output_wait(self, slot):
+        """sw2d_output_wait: copies of (h, vv, uu) of a snapshot."""
+        ptrs = [C.POINTER(C.c_double)() for _ in range(3)]
+        check(lib().sw2d_output_wait(self.h_, slot, *[C.byref(q) for q in ptrs]))
+        n = self.rows * self.cols
+        return [np.ctypeslib.as_array(q, shape=(n,)).reshape(self.rows, self.cols).copy() for q in ptrs]
 
     def read_noalloc(self, field):
         """sw2d_read_noalloc: h, U, V or z through the library's fixed device
