@@ -191,6 +191,14 @@ int sw2d_create(const sw2d_grid *grid, const sw2d_params *params, sw2d_ctx **out
   x->tiles_x = (C + sw - 1) / sw;
   x->TH = sw2d::default_seg_rows(R, x->tiles_x, sw2d::step_slots(x->prec == SW2D_FP64),
                                  x->prec == SW2D_FP64 ? 9 : 3, x->prec == SW2D_FP64 ? 64 : 160);
+  {
+    const int cap = sw2d::reach_rows_cap((long long)sw2d::STATE_ARRAYS * x->P * x->esz);
+    if (cap == 0) {
+      delete x;
+      return fail(SW2D_E_INVALID, "grid too wide");
+    }
+    x->TH = std::min(x->TH, cap);
+  }
   x->tiles_y = (R + x->TH - 1) / x->TH;
   x->n_tiles = x->tiles_x * x->tiles_y;
   x->n_acc = x->tiles_x * ((R + 3) / 4);

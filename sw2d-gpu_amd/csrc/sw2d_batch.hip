@@ -88,7 +88,7 @@ struct sw2d_batch {
       }
       for (size_t m : group) used[m] = 1;
       const sw2d_ctx *c = scen[ida];
-      const int th = rows[a];
+      const int th = std::min(rows[a], sw2d::reach_rows_cap((long long)c->RS * c->esz));
       int slot = 0;
       for (int i0 = 0; i0 < c->R; i0 += th)
         for (int j0 = 0; j0 < c->C; j0 += sw2d::strip_width(prec == SW2D_FP64), slot++)

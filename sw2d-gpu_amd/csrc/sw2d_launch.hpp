@@ -41,6 +41,16 @@ inline int default_seg_rows(int rows, int strips, long long slots, int rounds, i
   return (int)(s < 16 ? 16 : s > 4096 ? 4096 : s);
 }
 
+// Longest segment (a multiple of 4, at most 4096 rows) whose reach in the
+// state block -- its rows, the 4 above and the 5 below -- stays below 2^31
+// bytes for a block row of `row_bytes`: the step kernel addresses a segment's
+// rows with 32-bit offsets and drops stores at offset 2^31.  0 when not even
+// 4 rows fit.
+inline int reach_rows_cap(long long row_bytes) {
+  const long long n = ((1LL << 31) / (row_bytes > 0 ? row_bytes : 1) - 10) / 4 * 4;
+  return (int)(n < 4 ? 0 : n > 4096 ? 4096 : n);
+}
+
 // Segment rows of the scenarios of a batch whose live set holds
 // `strip_rows` strip rows in all (sum over scenarios of strips x rows): the
 // height that gives the whole set about twelve rounds of wave slots, at
