@@ -322,6 +322,11 @@ def main():
                     help="skip the FP32-vs-FP64 accuracy leg (config 2)")
     ap.add_argument("--ensemble-grids", type=int, default=64)
     ap.add_argument("--serving-requests", type=int, default=200)
+    ap.add_argument("--sync-every", type=int, default=0,
+                    help="run the warm-up iterations in chunks of this many, with a host "
+                         "synchronisation after each (for counter runs under rocprofv3 --pmc, "
+                         "where a single call enqueuing 10,000+ iterations has crashed the "
+                         "profiler's worker thread)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy 5 times (the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -378,7 +383,15 @@ def main():
     s = workloads.make_sim(w)
     cells = w["z"].size
 
-    s.step(args.warmup)
+    def warm(sim, n):
+        chunk = args.sync_every if args.sync_every > 0 else n
+        while n > 0:
+            k = min(chunk, n)
+            sim.step(k)
+            sim.sync()
+            n -= k
+
+    warm(s, args.warmup)
     s.sync()
     st0 = s.stats()
     log("warmup %d iterations done: t = %.2f s, D = %.4f s" % (args.warmup, st0["t"], st0["last_D"]))
@@ -400,7 +413,7 @@ def main():
     # regimes as the timed run
     s.close()
     s = workloads.make_sim(w)
-    s.step(args.warmup)
+    warm(s, args.warmup)
     ms, nprof, done = [0.0, 0.0, 0.0], 0, 0
     while done < args.steps:
         k = min(20, args.steps - done)
