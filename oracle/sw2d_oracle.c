@@ -77,10 +77,11 @@ double orc_pow_fric(double x) {
   double b = m + 1.0; /* b + bl = m + 1 exactly */
   double bv = b - m;
   double bl = (m - (b - bv)) + (1.0 - bv);
-  double s_hi = a / b;
+  double ib = 1.0 / b; /* one division: s_hi within an ulp of a/b, s_lo corrects it */
+  double s_hi = a * ib;
   double p = s_hi * b;
   double pe = fma(s_hi, b, -p);
-  double s_lo = (((a - p) - pe) - s_hi * bl) / b;
+  double s_lo = (((a - p) - pe) - s_hi * bl) * ib;
   double z = s_hi * s_hi;
   double P = ATANH_C[0];
   for (int j = 1; j < 12; j++) P = P * z + ATANH_C[j];

@@ -166,15 +166,19 @@ __host__ __device__ inline double pow_fric_f64(double x) {
     m = m * 2.0;
     e = e - 1;
   }
-  // s_hi + s_lo = (m - 1)/(m + 1); m - 1 is exact, b + bl = m + 1 exactly
+  // s_hi + s_lo = (m - 1)/(m + 1); m - 1 is exact, b + bl = m + 1 exactly.
+  // One division: s_hi = a (1/b) is within an ulp of a/b, the remainder
+  // a - s_hi b is exact, and s_lo carries it (its own relative error, 2^-52,
+  // is far below what the result keeps).
   const double a = m - 1.0;
   const double b = m + 1.0;
   const double bv = b - m;
   const double bl = (m - (b - bv)) + (1.0 - bv);
-  const double s_hi = a / b;
+  const double ib = 1.0 / b;
+  const double s_hi = a * ib;
   const double p = s_hi * b;
   const double pe = fma(s_hi, b, -p);
-  const double s_lo = (((a - p) - pe) - s_hi * bl) / b;
+  const double s_lo = (((a - p) - pe) - s_hi * bl) * ib;
   // log m = 2 s + 2 s^3 (1/3 + z/5 + ... + z^11/25), z = s^2 <= 0.0295
   const double z = s_hi * s_hi;
   double P = 0x1.47ae147ae147bp-5;  // 1/25
