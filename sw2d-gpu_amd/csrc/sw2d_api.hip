@@ -177,6 +177,10 @@ int sw2d_create(const sw2d_grid *grid, const sw2d_params *params, sw2d_ctx **out
     return fail(SW2D_E_INVALID, "precision must be SW2D_FP64 or SW2D_FP32");
   if (!(params->dt > 0)) return fail(SW2D_E_INVALID, "dt must be > 0");
   if (params->cfl < 0) return fail(SW2D_E_INVALID, "cfl must be >= 0");
+  // the FP32 form's friction power takes face depths above th: th must be a
+  // normal float, so that such depths are too
+  if (params->precision == SW2D_FP32 && !(params->th >= 1e-30))
+    return fail(SW2D_E_INVALID, "th must be >= 1e-30 in the FP32 form");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
     return fail(SW2D_E_DEVICE, "no HIP device available");
