@@ -139,3 +139,23 @@ def test_library_friction_pow_is_the_oracle_sequence():
     for _ in range(20000):
         x = 10 ** rnd.uniform(-4, 3)
         assert sw2d.friction_pow(x) == O.pow_fric(x), x
+
+
+def test_fp32_form_refuses_a_threshold_below_a_normal_float():
+    """sw2d_create checks its arguments before it looks for a device: the
+    FP32 form refuses th < 1e-30 (its friction power needs face depths above
+    th to be normal floats); with th = 1e-4 the same call passes the checks
+    and, on a machine without a GPU, fails only for the missing device."""
+    z = np.zeros((8, 8))
+    with pytest.raises(sw2d.SW2DError) as e:
+        sw2d.Sim(z, z, 1.0, precision=sw2d.FP32, th=0.0)
+    assert e.value.code == sw2d.E_INVALID and "th" in str(e.value)
+    try:
+        import torch
+        has_gpu = torch.cuda.is_available()
+    except ImportError:
+        has_gpu = False
+    if not has_gpu:
+        with pytest.raises(sw2d.SW2DError) as e:
+            sw2d.Sim(z, z, 1.0, precision=sw2d.FP32, th=1e-4)
+        assert e.value.code == sw2d.E_DEVICE
