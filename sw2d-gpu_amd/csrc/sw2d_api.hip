@@ -194,7 +194,7 @@ int sw2d_create(const sw2d_grid *grid, const sw2d_params *params, sw2d_ctx **out
   const int sw = sw2d::strip_width(x->prec == SW2D_FP64);
   x->tiles_x = (C + sw - 1) / sw;
   x->TH = sw2d::default_seg_rows(R, x->tiles_x, sw2d::step_slots(x->prec == SW2D_FP64),
-                                 x->prec == SW2D_FP64 ? 9 : 3, x->prec == SW2D_FP64 ? 64 : 160);
+                                 x->prec == SW2D_FP64 ? 6 : 3, x->prec == SW2D_FP64 ? 64 : 160);
   {
     const int cap = sw2d::reach_rows_cap((long long)sw2d::STATE_ARRAYS * x->P * x->esz);
     if (cap == 0) {

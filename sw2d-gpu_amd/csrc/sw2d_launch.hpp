@@ -18,9 +18,12 @@ namespace sw2d {
 //    MI355X, took 100 rows (2.94 rounds) to 115.0 Gcell/s, 76 rows (3.9
 //    rounds) to 114.2 and 152 rows (1.94 rounds) to 112.0; 296 rows (one
 //    round) and 32 rows lost 3-6 % and 4 % on an earlier form of the step;
-//  * FP64 (2,048 slots), nine rounds: 64 rows gave 42.6 Gcell/s, 32 rows
-//    41.8, 148 rows 38.3 and 296 rows (two rounds) 34.7; the FP64 rows are
-//    twice as wide and long segments lose their halo reuse in L2.
+//  * FP64 (3,072 slots), six rounds: 64 rows gave 42.6 Gcell/s, 32 rows
+//    41.8, 148 rows 38.3 and 296 rows 34.7 on an earlier form of the step;
+//    on the current one, from a wet state, 64 rows 2.024 ms per launch, 32
+//    rows 2.087, 44 rows 2.065, 88 rows 2.098, 128 rows 2.032 and 200 rows
+//    2.139; the FP64 rows are twice as wide and long segments lose their
+//    halo reuse in L2.
 // On larger grids the number of rounds grows until the segments are no
 // longer than `max_rows` (160 FP32, 64 FP64), the longest measured to pay.
 // SW2D_SEG_ROWS in the environment overrides it (a multiple of 4 in
@@ -71,14 +74,14 @@ inline int batch_seg_rows(long long strip_rows, long long slots, int max_rows) {
 }
 
 // concurrent single-wavefront workgroups of the step kernel on the current
-// device: CUs x 4 SIMDs x waves per SIMD (4 for the FP32 kernel, 2 for FP64,
+// device: CUs x 4 SIMDs x waves per SIMD (4 for the FP32 kernel, 3 for FP64,
 // set by their register use)
 inline long long step_slots(bool fp64) {
   int dev = 0, cus = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
     cus = 256;
-  return (long long)cus * 4 * (fp64 ? 2 : 4);
+  return (long long)cus * 4 * (fp64 ? 3 : 4);
 }
 
 struct IterLaunch {
