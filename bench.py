@@ -330,6 +330,9 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy 5 times (the reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="only run the batched-ensemble leg (config 4) and print its record "
+                         "(for counter runs of the batch)")
     ap.add_argument("--stream-ceiling", action="store_true",
                     help="only time plain streaming kernels (torch's device copy and a two-input "
                          "add, 1 and 4 GiB per array) to give the HBM rate such kernels reach")
@@ -357,6 +360,9 @@ def main():
             del a, b, c
             torch.cuda.empty_cache()
         print(json.dumps(out), flush=True)
+        return
+    if args.ensemble_only:
+        print(json.dumps(ensemble_bench(args.ensemble_grids)), flush=True)
         return
     if args.copy_calibration:
         import torch
