@@ -226,6 +226,14 @@ int sw2d_batch_scenario(sw2d_batch *b, int id, sw2d_ctx **ctx);
 int sw2d_batch_touch(sw2d_batch *b);
 /* Removes a scenario from the work-list and frees its device memory. */
 int sw2d_batch_remove(sw2d_batch *b, int id);
+/* Takes a scenario out of the work-list without destroying it: *ctx_out is
+ * its context, owned by the caller from then on (sw2d_destroy frees it),
+ * with its state, clock and accumulators as they are; it can be read, or
+ * stepped on its own.  Waits for the batch stream only: no device memory
+ * is freed, so other streams are not stalled (freeing device memory waits
+ * for the whole device).  Refused (SW2D_E_STATE) for a scenario whose bed
+ * and raster belong to another scenario, or are shared by one. */
+int sw2d_batch_detach(sw2d_batch *b, int id, sw2d_ctx **ctx_out);
 /* Segment length (rows) of one scenario's work items: a multiple of 4 in
  * [4, 4096]; 0 = the autotuned choice for its size bucket, or before any
  * tuning the batch default: the scenario's single-grid length raised to the

@@ -271,6 +271,24 @@ int sw2d_batch_remove(sw2d_batch *b, int id) {
   return SW2D_OK;
 }
 
+int sw2d_batch_detach(sw2d_batch *b, int id, sw2d_ctx **ctx_out) {
+  if (!b || !ctx_out || id < 0 || id >= (int)b->scen.size() || !b->scen[id])
+    return fail(SW2D_E_INVALID, "scenario id");
+  if (b->share_of[id] >= 0) return fail(SW2D_E_STATE, "the scenario reads another scenario's statics");
+  for (size_t k = 0; k < b->scen.size(); k++)
+    if (b->scen[k] && (int)k != id && b->share_of[k] == id)
+      return fail(SW2D_E_STATE, "another scenario shares this scenario's statics");
+  // launches still in flight read the scenario's arrays through the batch's
+  // work list: the context leaves the batch once they are done
+  HIPCHK(hipStreamSynchronize(b->stream));
+  sw2d_ctx *c = b->scen[id];
+  c->owner = nullptr;
+  b->scen[id] = nullptr;
+  b->dirty = true;
+  *ctx_out = c;
+  return SW2D_OK;
+}
+
 int sw2d_batch_set_tile_rows(sw2d_batch *b, int id, int rows) {
   if (!b || id < 0 || id >= (int)b->scen.size() || !b->scen[id]) return fail(SW2D_E_INVALID, "scenario id");
   if (rows != 0 && (rows < 4 || rows % 4 != 0 || rows > 4096))

@@ -14,7 +14,11 @@
 // class's stream is idle its finished scenarios are retired (their final
 // fields and statistics move to host memory, their device memory returns to
 // the budget), queued requests of that class are admitted and its next chunk
-// is enqueued.
+// is enqueued.  A retired scenario leaves its batch without being destroyed:
+// freeing device memory waits for the whole device, which would stall the
+// other classes' work, so retired contexts are kept (their memory still
+// charged to the budget) and destroyed together when an admission needs the
+// memory, or with the server.
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -80,6 +84,18 @@ struct sw2d_server {
   std::vector<Request> req;
   std::deque<int> queue;
   size_t used = 0;
+  // retired scenarios not yet destroyed, and their footprints
+  std::vector<std::pair<sw2d_ctx *, size_t>> retired;
+
+  // Destroys the retired contexts (one device-wide wait for all of them)
+  // and returns their memory to the budget.
+  void free_retired() {
+    for (auto &p : retired) {
+      sw2d_destroy(p.first);
+      used -= p.second;
+    }
+    retired.clear();
+  }
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 
   double now() const {
@@ -96,6 +112,7 @@ struct sw2d_server {
     while (!queue.empty()) {
       Request &r = req[queue.front()];
       if (max_slots > 0 && n_running() >= max_slots) break;
+      if (used + r.footprint > budget && !retired.empty()) free_retired();
       if (used + r.footprint > budget) break;
       if (!idle[r.cls]) break;
       Class &c = cls[r.cls];
@@ -162,12 +179,13 @@ struct sw2d_server {
       if ((rc = sw2d_read(ctx, SW2D_FIELD_H, r.H.data())) || (rc = sw2d_read(ctx, SW2D_FIELD_U, r.U.data())) ||
           (rc = sw2d_read(ctx, SW2D_FIELD_V, r.V.data())) || (rc = sw2d_get_stats(ctx, &r.stats)))
         return rc;
-      if ((rc = sw2d_batch_remove(c.batch, r.scen))) return rc;
+      sw2d_ctx *gone = nullptr;
+      if ((rc = sw2d_batch_detach(c.batch, r.scen, &gone))) return rc;
+      retired.emplace_back(gone, r.footprint);  // still charged to the budget
       r.iterations = iters[r.scen];
       r.scen = -1;
       r.state = SW2D_REQ_DONE;
       r.finish_s = t;
-      used -= r.footprint;
     }
     c.running.swap(still);
     return SW2D_OK;
@@ -229,6 +247,7 @@ int sw2d_server_create(int precision, size_t mem_budget, int max_slots, sw2d_ser
 
 void sw2d_server_destroy(sw2d_server *s) {
   if (!s) return;
+  s->free_retired();
   for (Class &c : s->cls)
     if (c.batch) sw2d_batch_destroy(c.batch);
   delete s;

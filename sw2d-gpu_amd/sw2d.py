@@ -31,7 +31,7 @@ EXPORTS = [
     "sw2d_run_db", "sw2d_profile_begin", "sw2d_profile_end",
     "sw2d_gen_fractal_dem", "sw2d_gen_tilt", "sw2d_gen_uniform",
     "sw2d_gen_lognormal", "sw2d_gen_gamma", "sw2d_batch_create", "sw2d_batch_destroy",
-    "sw2d_batch_add", "sw2d_batch_scenario", "sw2d_batch_touch", "sw2d_batch_remove",
+    "sw2d_batch_add", "sw2d_batch_scenario", "sw2d_batch_touch", "sw2d_batch_remove", "sw2d_batch_detach",
     "sw2d_batch_set_tile_rows", "sw2d_batch_autotune", "sw2d_batch_tile_rows",
     "sw2d_batch_step", "sw2d_batch_sync", "sw2d_batch_status", "sw2d_batch_idle", "sw2d_device_memory",
     "sw2d_server_create", "sw2d_server_destroy", "sw2d_server_submit", "sw2d_server_advance",
@@ -145,6 +145,7 @@ def lib():
         L.sw2d_batch_scenario.argtypes = [vp, C.c_int, C.POINTER(vp)]
         L.sw2d_batch_touch.argtypes = [vp]
         L.sw2d_batch_remove.argtypes = [vp, C.c_int]
+        L.sw2d_batch_detach.argtypes = [vp, C.c_int, C.POINTER(vp)]
         L.sw2d_batch_set_tile_rows.argtypes = [vp, C.c_int, C.c_int]
         L.sw2d_batch_autotune.argtypes = [vp, C.c_int]
         L.sw2d_batch_tile_rows.argtypes = [vp, C.c_int, ip]
@@ -478,6 +479,14 @@ class Batch:
 
     def remove(self, sid):
         check(lib().sw2d_batch_remove(self.h_, sid))
+
+    def detach(self, sid):
+        """Takes the scenario out of the batch; returns its context as an
+        owning handle (closing it destroys the context)."""
+        h = C.c_void_p()
+        check(lib().sw2d_batch_detach(self.h_, sid, C.byref(h)))
+        r, c = self.shapes[sid]
+        return Sim.from_handle(h, r, c, owned=True)
 
     def set_tile_rows(self, sid, rows):
         check(lib().sw2d_batch_set_tile_rows(self.h_, sid, rows))

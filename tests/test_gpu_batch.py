@@ -178,3 +178,46 @@ def test_batch_scenario_context_is_not_stepped_alone():
     assert e.value.code == sw2d.E_STATE
     b.step(3)
     assert b.status()[1][sid] == 3
+
+
+def test_detach_continues_alone_bitwise():
+    """A scenario taken out of the batch with sw2d_batch_detach while
+    iterations are in flight keeps its state, clock and accumulators: stepped
+    on its own afterwards it equals a run that was alone throughout, bitwise;
+    the scenarios left in the batch go on; a scenario whose statics another
+    shares is refused."""
+    say("case: detach a batch scenario mid-run, continue it alone")
+    kw = dict(dt=0.2, manning=0.03, cfl=0.45, precision=sw2d.FP32, forcing_mode=sw2d.FORCING_STEP,
+              edge_open=sw2d.EDGE_SOUTH)
+    b = sw2d.Batch(sw2d.FP32)
+    shapes = [(160, 230), (210, 120)]
+    ids = []
+    for k, shp in enumerate(shapes):
+        z, h, n, rain = scen_inputs(40 + k, shp)
+        sid = b.add(z, h, 2.0, n_cell=n, **kw)
+        b.scenario(sid).set_rain(rain, 300.0)
+        ids.append(sid)
+    z0, h0, n0, _ = scen_inputs(40, shapes[0])
+    sib = b.add(z0, h0, 2.0, n_cell=n0, share_with=ids[0], **kw)
+    b.touch()
+    b.step(120)                      # asynchronous: graph replays in flight
+    with pytest.raises(sw2d.SW2DError) as e:
+        b.detach(ids[0])             # its statics are shared by `sib`
+    assert e.value.code == sw2d.E_STATE
+    d = b.detach(ids[1])
+    d.step(80)
+    b.step(30)
+    z, h, n, rain = scen_inputs(41, shapes[1])
+    a = sw2d.Sim(z, h, 2.0, n_cell=n, **kw)
+    a.set_rain(rain, 300.0)
+    a.step(200)
+    sa, sd = a.stats(), d.stats()
+    assert sa["iterations"] == sd["iterations"] == 200 and sa["t"] == sd["t"]
+    for f in (sw2d.F_H, sw2d.F_U, sw2d.F_V):
+        assert np.array_equal(a.read(f), d.read(f)), f
+    # volume sums: per-segment partial sums, so the segment heights (batch
+    # and alone differ) change their last bits only
+    for k in ("rain_in", "outflow"):
+        assert abs(sa[k] - sd[k]) <= 1e-10 * max(1.0, abs(sa[k])), k
+    d.close()
+    assert b.status()[1][ids[0]] == 150 and b.status()[1][sib] == 150
