@@ -234,6 +234,12 @@ int sw2d_batch_remove(sw2d_batch *b, int id);
  * for the whole device).  Refused (SW2D_E_STATE) for a scenario whose bed
  * and raster belong to another scenario, or are shared by one. */
 int sw2d_batch_detach(sw2d_batch *b, int id, sw2d_ctx **ctx_out);
+/* Adds an existing context (one not in a batch, of the batch's precision)
+ * as a scenario, with its state, clock, forcing and accumulators as they
+ * are; the batch owns it from then on.  It joins the work-list at the next
+ * sw2d_batch_step: nothing waits for the batch stream, so a context can be
+ * created and given its forcing while the batch runs. */
+int sw2d_batch_attach(sw2d_batch *b, sw2d_ctx *ctx, int *id_out);
 /* Segment length (rows) of one scenario's work items: a multiple of 4 in
  * [4, 4096]; 0 = the autotuned choice for its size bucket, or before any
  * tuning the batch default: the scenario's single-grid length raised to the

@@ -289,6 +289,25 @@ int sw2d_batch_detach(sw2d_batch *b, int id, sw2d_ctx **ctx_out) {
   return SW2D_OK;
 }
 
+int sw2d_batch_attach(sw2d_batch *b, sw2d_ctx *c, int *id_out) {
+  if (!b || !c || !id_out) return fail(SW2D_E_INVALID, "null argument");
+  if (c->owner) return fail(SW2D_E_STATE, "the context is in a batch already");
+  if (c->prec != b->prec) return fail(SW2D_E_INVALID, "scenario precision differs from the batch");
+  // work the context may still have on its own stream (it may have been
+  // stepped alone) ends before the batch reads its state
+  HIPCHK(hipStreamSynchronize(c->stream));
+  // the batch's device lists are rewritten only by the next rebuild, which
+  // waits for the batch stream: nothing here touches them
+  c->owner = b->stream;
+  b->scen.push_back(c);
+  b->share_of.push_back(-1);
+  b->rows_override.push_back(0);
+  b->parked.push_back(0);
+  b->dirty = true;
+  *id_out = (int)b->scen.size() - 1;
+  return SW2D_OK;
+}
+
 int sw2d_batch_set_tile_rows(sw2d_batch *b, int id, int rows) {
   if (!b || id < 0 || id >= (int)b->scen.size() || !b->scen[id]) return fail(SW2D_E_INVALID, "scenario id");
   if (rows != 0 && (rows < 4 || rows % 4 != 0 || rows > 4096))

@@ -6,18 +6,20 @@
 // Requests wait in a FIFO queue with host copies of their inputs.  Admission
 // takes the head of the queue while the admitted scenarios' device
 // footprint stays within the budget and a slot is free (strict arrival
-// order: a request that does not fit, or whose class is still running its
-// last chunk, blocks those behind it).  Each size class is a sw2d_batch with
-// its own stream: a chunk of iterations of one class is enqueued without
-// waiting, so the classes' kernels run concurrently and a class of small
-// grids advances at its own launch rate instead of the large grids'.  When a
-// class's stream is idle its finished scenarios are retired (their final
-// fields and statistics move to host memory, their device memory returns to
-// the budget), queued requests of that class are admitted and its next chunk
-// is enqueued.  A retired scenario leaves its batch without being destroyed:
-// freeing device memory waits for the whole device, which would stall the
-// other classes' work, so retired contexts are kept (their memory still
-// charged to the budget) and destroyed together when an admission needs the
+// order: a request that does not fit blocks those behind it).  Each size
+// class is a sw2d_batch with its own stream: a chunk of iterations of one
+// class is enqueued without waiting, so the classes' kernels run
+// concurrently and a class of small grids advances at its own launch rate
+// instead of the large grids'.  When a class's stream is idle its finished
+// scenarios leave the batch (sw2d_batch_detach) and its next chunk is
+// enqueued at once; then, while the chunks run, queued requests are
+// admitted (each context created on its own stream and attached to its
+// class's batch, joining the work-list at that class's next chunk) and the
+// finished scenarios' final fields and statistics are read into host
+// memory.  A retired scenario's context is not destroyed at once: freeing
+// device memory waits for the whole device, which would stall the other
+// classes' work, so retired contexts are kept (their memory still charged
+// to the budget) and destroyed together when an admission needs the
 // memory, or with the server.
 #include <algorithm>
 #include <chrono>
@@ -107,14 +109,18 @@ struct sw2d_server {
     return n;
   }
 
-  // Admit from the head of the queue into idle classes.
-  int admit(const bool *idle, bool *touched) {
+  // Admit from the head of the queue (strict arrival order) while the
+  // footprint fits the budget and a slot is free.  A request's context is
+  // created on its own stream and given its rain before it joins its class's
+  // batch (sw2d_batch_attach waits for no batch stream): the uploads run
+  // while the classes' chunks do, and the scenario enters the work-list at
+  // its class's next chunk.
+  int admit() {
     while (!queue.empty()) {
       Request &r = req[queue.front()];
       if (max_slots > 0 && n_running() >= max_slots) break;
       if (used + r.footprint > budget && !retired.empty()) free_retired();
       if (used + r.footprint > budget) break;
-      if (!idle[r.cls]) break;
       Class &c = cls[r.cls];
       sw2d_grid g;
       g.rows = r.rows;
@@ -125,20 +131,19 @@ struct sw2d_server {
       g.z = r.z.data();
       g.h0 = r.h0.data();
       g.n_cell = r.has_n ? r.n.data() : nullptr;
-      int id = -1;
-      int rc = sw2d_batch_add(c.batch, &g, &r.params, -1, &id);
-      if (rc) return rc;
-      c.n_added = std::max(c.n_added, id + 1);
       sw2d_ctx *ctx = nullptr;
-      rc = sw2d_batch_scenario(c.batch, id, &ctx);
-      if (!rc && !r.rain.empty())
+      int rc = sw2d_create(&g, &r.params, &ctx);
+      if (rc) return rc;
+      if (!r.rain.empty())
         rc = sw2d_set_rain(ctx, r.rain.data(), (int)r.rain.size(), r.dtrain, nullptr, 0);
+      int id = -1;
+      if (!rc) rc = sw2d_batch_attach(c.batch, ctx, &id);
       if (rc) {
-        // the scenario is in the batch but not admitted: take it out again
-        // (the request stays at the head of the queue)
-        (void)sw2d_batch_remove(c.batch, id);
+        // not admitted: the request stays at the head of the queue
+        sw2d_destroy(ctx);
         return rc;
       }
+      c.n_added = std::max(c.n_added, id + 1);
       r.scen = id;
       r.state = SW2D_REQ_RUNNING;
       r.admit_s = now();
@@ -148,13 +153,13 @@ struct sw2d_server {
       used += r.footprint;
       c.running.push_back(queue.front());
       queue.pop_front();
-      touched[r.cls] = true;
     }
     return SW2D_OK;
   }
 
-  // Retire the finished scenarios of an idle class.
-  int retire(int k) {
+  // Takes the finished scenarios of an idle class out of its batch; their
+  // results are read by finish() once the next chunks are enqueued.
+  int detach_finished(int k, std::vector<std::pair<int, sw2d_ctx *>> &out) {
     Class &c = cls[k];
     if (c.running.empty()) return SW2D_OK;
     int n = 0;
@@ -162,7 +167,6 @@ struct sw2d_server {
     std::vector<int64_t> iters(std::max(c.n_added, 1));
     int rc = sw2d_batch_status(c.batch, done.data(), iters.data(), c.n_added, &n);
     if (rc) return rc;
-    const double t = now();
     std::vector<int> still;
     for (int q : c.running) {
       Request &r = req[q];
@@ -171,31 +175,42 @@ struct sw2d_server {
         continue;
       }
       sw2d_ctx *ctx = nullptr;
-      rc = sw2d_batch_scenario(c.batch, r.scen, &ctx);
-      if (rc) return rc;
-      r.H.resize((size_t)r.rows * r.cols);
-      r.U.resize((size_t)(r.rows + 1) * r.cols);
-      r.V.resize((size_t)r.rows * (r.cols + 1));
-      if ((rc = sw2d_read(ctx, SW2D_FIELD_H, r.H.data())) || (rc = sw2d_read(ctx, SW2D_FIELD_U, r.U.data())) ||
-          (rc = sw2d_read(ctx, SW2D_FIELD_V, r.V.data())) || (rc = sw2d_get_stats(ctx, &r.stats)))
-        return rc;
-      sw2d_ctx *gone = nullptr;
-      if ((rc = sw2d_batch_detach(c.batch, r.scen, &gone))) return rc;
-      retired.emplace_back(gone, r.footprint);  // still charged to the budget
+      if ((rc = sw2d_batch_detach(c.batch, r.scen, &ctx))) return rc;
       r.iterations = iters[r.scen];
       r.scen = -1;
-      r.state = SW2D_REQ_DONE;
-      r.finish_s = t;
+      out.emplace_back(q, ctx);
     }
     c.running.swap(still);
     return SW2D_OK;
   }
 
+  // Reads the final fields and statistics of detached scenarios into host
+  // memory: the requests are done from then on.  Their contexts are kept
+  // (memory still charged) until free_retired().
+  int finish(std::vector<std::pair<int, sw2d_ctx *>> &list) {
+    int rc = SW2D_OK;
+    for (auto &qc : list) {
+      Request &r = req[qc.first];
+      sw2d_ctx *ctx = qc.second;
+      retired.emplace_back(ctx, r.footprint);
+      if (rc) continue;  // after a failure the rest are left unread, not done
+      r.H.resize((size_t)r.rows * r.cols);
+      r.U.resize((size_t)(r.rows + 1) * r.cols);
+      r.V.resize((size_t)r.rows * (r.cols + 1));
+      if ((rc = sw2d_read(ctx, SW2D_FIELD_H, r.H.data())) || (rc = sw2d_read(ctx, SW2D_FIELD_U, r.U.data())) ||
+          (rc = sw2d_read(ctx, SW2D_FIELD_V, r.V.data())) || (rc = sw2d_get_stats(ctx, &r.stats)))
+        continue;
+      r.state = SW2D_REQ_DONE;
+      r.finish_s = now();
+    }
+    list.clear();
+    return rc;
+  }
+
   int advance(int64_t n_iter) {
-    bool idle[N_CLASS], touched[N_CLASS] = {};
-    // wait until a class with running scenarios has finished its last chunk,
-    // or the request at the head of the queue can go into an idle class (or
-    // nothing runs at all)
+    bool idle[N_CLASS];
+    // wait until a class with running scenarios has finished its last
+    // chunk, or the request at the head of the queue fits (or nothing runs)
     for (;;) {
       bool ready = false, busy = false;
       for (int k = 0; k < N_CLASS; k++) {
@@ -206,24 +221,32 @@ struct sw2d_server {
         if (!idle[k]) busy = true;
         else if (!cls[k].running.empty()) ready = true;
       }
-      if (!queue.empty() && idle[req[queue.front()].cls]) ready = true;
+      if (!queue.empty() && (max_slots == 0 || n_running() < max_slots) &&
+          (used + req[queue.front()].footprint <= budget || !retired.empty()))
+        ready = true;
       if (ready || !busy) break;
       std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
+    // 1. the idle classes' finished scenarios leave their batches
+    std::vector<std::pair<int, sw2d_ctx *>> finished;
     for (int k = 0; k < N_CLASS; k++)
       if (idle[k]) {
-        int rc = retire(k);
-        if (rc) return rc;
+        int rc = detach_finished(k, finished);
+        if (rc) {
+          (void)finish(finished);
+          return rc;
+        }
       }
-    int rc = admit(idle, touched);
-    if (rc) return rc;
-    for (int k = 0; k < N_CLASS; k++) {
-      if (!idle[k]) continue;
+    // 2. their next chunks go to the device
+    int rc = SW2D_OK;
+    for (int k = 0; k < N_CLASS && !rc; k++) {
       Class &c = cls[k];
-      if (touched[k] && (rc = sw2d_batch_touch(c.batch))) return rc;
-      if (!c.running.empty() && n_iter > 0 && (rc = sw2d_batch_step(c.batch, n_iter * CLASS_MULT[k]))) return rc;
+      if (idle[k] && !c.running.empty() && n_iter > 0) rc = sw2d_batch_step(c.batch, n_iter * CLASS_MULT[k]);
     }
-    return SW2D_OK;
+    // 3. while they run: admissions and the finished scenarios' read-back
+    if (!rc) rc = admit();
+    const int rf = finish(finished);
+    return rc ? rc : rf;
   }
 };
 

@@ -31,7 +31,7 @@ EXPORTS = [
     "sw2d_run_db", "sw2d_profile_begin", "sw2d_profile_end",
     "sw2d_gen_fractal_dem", "sw2d_gen_tilt", "sw2d_gen_uniform",
     "sw2d_gen_lognormal", "sw2d_gen_gamma", "sw2d_batch_create", "sw2d_batch_destroy",
-    "sw2d_batch_add", "sw2d_batch_scenario", "sw2d_batch_touch", "sw2d_batch_remove", "sw2d_batch_detach",
+    "sw2d_batch_add", "sw2d_batch_scenario", "sw2d_batch_touch", "sw2d_batch_remove", "sw2d_batch_detach", "sw2d_batch_attach",
     "sw2d_batch_set_tile_rows", "sw2d_batch_autotune", "sw2d_batch_tile_rows",
     "sw2d_batch_step", "sw2d_batch_sync", "sw2d_batch_status", "sw2d_batch_idle", "sw2d_device_memory",
     "sw2d_server_create", "sw2d_server_destroy", "sw2d_server_submit", "sw2d_server_advance",
@@ -146,6 +146,7 @@ def lib():
         L.sw2d_batch_touch.argtypes = [vp]
         L.sw2d_batch_remove.argtypes = [vp, C.c_int]
         L.sw2d_batch_detach.argtypes = [vp, C.c_int, C.POINTER(vp)]
+        L.sw2d_batch_attach.argtypes = [vp, vp, ip]
         L.sw2d_batch_set_tile_rows.argtypes = [vp, C.c_int, C.c_int]
         L.sw2d_batch_autotune.argtypes = [vp, C.c_int]
         L.sw2d_batch_tile_rows.argtypes = [vp, C.c_int, ip]
@@ -487,6 +488,17 @@ class Batch:
         check(lib().sw2d_batch_detach(self.h_, sid, C.byref(h)))
         r, c = self.shapes[sid]
         return Sim.from_handle(h, r, c, owned=True)
+
+    def attach(self, sim):
+        """Adds an existing context (a Sim that owns it) as a scenario; the
+        batch owns the context from then on.  Returns the scenario id."""
+        out = C.c_int()
+        check(lib().sw2d_batch_attach(self.h_, sim.h_, C.byref(out)))
+        sim.owned = False
+        while len(self.shapes) <= out.value:
+            self.shapes.append(None)
+        self.shapes[out.value] = (sim.rows, sim.cols)
+        return out.value
 
     def set_tile_rows(self, sid, rows):
         check(lib().sw2d_batch_set_tile_rows(self.h_, sid, rows))
