@@ -221,3 +221,35 @@ def test_detach_continues_alone_bitwise():
         assert abs(sa[k] - sd[k]) <= 1e-10 * max(1.0, abs(sa[k])), k
     d.close()
     assert b.status()[1][ids[0]] == 150 and b.status()[1][sib] == 150
+
+
+def test_attach_a_stepped_context():
+    """A context stepped on its own, then attached to a running batch with
+    sw2d_batch_attach, continues from its state: after more batch steps it
+    equals a run that was alone throughout, bitwise; a context already in a
+    batch is refused."""
+    say("case: attach a context stepped alone to a running batch")
+    kw = dict(dt=0.2, manning=0.03, cfl=0.45, precision=sw2d.FP32, forcing_mode=sw2d.FORCING_STEP)
+    b = sw2d.Batch(sw2d.FP32)
+    z, h, n, rain = scen_inputs(50, (150, 170))
+    first = b.add(z, h, 2.0, n_cell=n, **kw)
+    b.scenario(first).set_rain(rain, 300.0)
+    b.touch()
+    b.step(64)                       # asynchronous: in flight during the attach
+    z2, h2, n2, rain2 = scen_inputs(51, (190, 110))
+    c = sw2d.Sim(z2, h2, 2.0, n_cell=n2, **kw)
+    c.set_rain(rain2, 300.0)
+    c.step(45)
+    sid = b.attach(c)
+    with pytest.raises(sw2d.SW2DError) as e:
+        b.attach(b.scenario(first))  # already a batch scenario
+    assert e.value.code == sw2d.E_STATE
+    b.step(70)
+    a = sw2d.Sim(z2, h2, 2.0, n_cell=n2, **kw)
+    a.set_rain(rain2, 300.0)
+    a.step(115)
+    s = b.scenario(sid)
+    assert s.stats()["iterations"] == 115 and s.stats()["t"] == a.stats()["t"]
+    for f in (sw2d.F_H, sw2d.F_U, sw2d.F_V):
+        assert np.array_equal(a.read(f), s.read(f)), f
+    assert b.status()[1][first] == 134
