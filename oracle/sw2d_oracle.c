@@ -84,7 +84,7 @@ double orc_pow_fric(double x) {
   double s_lo = (((a - p) - pe) - s_hi * bl) * ib;
   double z = s_hi * s_hi;
   double P = ATANH_C[0];
-  for (int j = 1; j < 12; j++) P = P * z + ATANH_C[j];
+  for (int j = 1; j < 12; j++) P = fma(P, z, ATANH_C[j]); /* Horner steps fused */
   double L_hi = 2.0 * s_hi;
   double L_lo = 2.0 * s_lo + 2.0 * (s_hi * z) * P;
   double E_hi = (double)e * LN2_HI; /* exact */
@@ -103,7 +103,7 @@ double orc_pow_fric(double x) {
   double r = r_hi + r_lo;
   double r_e = r_lo - (r - r_hi);
   double q = EXP_C[0];
-  for (int j = 1; j < 14; j++) q = q * r + EXP_C[j];
+  for (int j = 1; j < 14; j++) q = fma(q, r, EXP_C[j]);
   q = (r * r) * q;
   double one_r = 1.0 + r;
   double one_e = (1.0 - one_r) + r;
