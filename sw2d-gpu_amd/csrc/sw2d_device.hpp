@@ -145,6 +145,17 @@ __host__ __device__ inline double forcing_value(double time, const double *table
   return table[i0] + (table[i1] - table[i0]) / (dT * (time - dT * (it)));
 }
 
+// fma(a, b, k) for a constant k: one correctly rounded fused multiply-add on
+// either side.  On the device k is passed in a scalar register pair (two
+// scalar moves): left to itself the compiler builds each such constant in a
+// vector register pair for v_fmac_f64, two vector moves per Horner step.
+static __device__ inline double fma_k(double a, double b, double k) {
+  double r;
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(k));
+  return r;
+}
+static __host__ inline double fma_k(double a, double b, double k) { return fma(a, b, k); }
+
 // x^1.3333330 (the friction exponent of L:195, L:312) for 0 < x < 1e300,
 // from IEEE operations only: +, -, *, / and fma (each correctly rounded),
 // frexp, ldexp and rint (exact).  The same sequence therefore gives the same
@@ -185,17 +196,17 @@ static __host__ __device__ __attribute__((noinline)) double pow_fric_f64(double 
   // (Horner steps fused, here and in the exponential below)
   const double z = s_hi * s_hi;
   double P = 0x1.47ae147ae147bp-5;  // 1/25
-  P = fma(P, z, 0x1.642c8590b2164p-5);  // 1/23
-  P = fma(P, z, 0x1.8618618618618p-5);  // 1/21
-  P = fma(P, z, 0x1.af286bca1af28p-5);  // 1/19
-  P = fma(P, z, 0x1.e1e1e1e1e1e1ep-5);  // 1/17
-  P = fma(P, z, 0x1.1111111111111p-4);  // 1/15
-  P = fma(P, z, 0x1.3b13b13b13b14p-4);  // 1/13
-  P = fma(P, z, 0x1.745d1745d1746p-4);  // 1/11
-  P = fma(P, z, 0x1.c71c71c71c71cp-4);  // 1/9
-  P = fma(P, z, 0x1.2492492492492p-3);  // 1/7
-  P = fma(P, z, 0x1.999999999999ap-3);  // 1/5
-  P = fma(P, z, 0x1.5555555555555p-2);  // 1/3
+  P = fma_k(P, z, 0x1.642c8590b2164p-5);  // 1/23
+  P = fma_k(P, z, 0x1.8618618618618p-5);  // 1/21
+  P = fma_k(P, z, 0x1.af286bca1af28p-5);  // 1/19
+  P = fma_k(P, z, 0x1.e1e1e1e1e1e1ep-5);  // 1/17
+  P = fma_k(P, z, 0x1.1111111111111p-4);  // 1/15
+  P = fma_k(P, z, 0x1.3b13b13b13b14p-4);  // 1/13
+  P = fma_k(P, z, 0x1.745d1745d1746p-4);  // 1/11
+  P = fma_k(P, z, 0x1.c71c71c71c71cp-4);  // 1/9
+  P = fma_k(P, z, 0x1.2492492492492p-3);  // 1/7
+  P = fma_k(P, z, 0x1.999999999999ap-3);  // 1/5
+  P = fma_k(P, z, 0x1.5555555555555p-2);  // 1/3
   const double L_hi = 2.0 * s_hi;
   const double L_lo = 2.0 * s_lo + 2.0 * (s_hi * z) * P;
   // log x = e ln2 + log m (e LN2_HI is exact)
@@ -217,19 +228,19 @@ static __host__ __device__ __attribute__((noinline)) double pow_fric_f64(double 
   const double r = r_hi + r_lo;
   const double r_e = r_lo - (r - r_hi);
   double q = 0x1.ae7f3e733b81fp-41;  // 1/15!
-  q = fma(q, r, 0x1.93974a8c07c9dp-37);  // 1/14!
-  q = fma(q, r, 0x1.6124613a86d09p-33);  // 1/13!
-  q = fma(q, r, 0x1.1eed8eff8d898p-29);  // 1/12!
-  q = fma(q, r, 0x1.ae64567f544e4p-26);  // 1/11!
-  q = fma(q, r, 0x1.27e4fb7789f5cp-22);  // 1/10!
-  q = fma(q, r, 0x1.71de3a556c734p-19);  // 1/9!
-  q = fma(q, r, 0x1.a01a01a01a01ap-16);  // 1/8!
-  q = fma(q, r, 0x1.a01a01a01a01ap-13);  // 1/7!
-  q = fma(q, r, 0x1.6c16c16c16c17p-10);  // 1/6!
-  q = fma(q, r, 0x1.1111111111111p-7);   // 1/5!
-  q = fma(q, r, 0x1.5555555555555p-5);   // 1/4!
-  q = fma(q, r, 0x1.5555555555555p-3);   // 1/3!
-  q = fma(q, r, 0x1.0000000000000p-1);   // 1/2!
+  q = fma_k(q, r, 0x1.93974a8c07c9dp-37);  // 1/14!
+  q = fma_k(q, r, 0x1.6124613a86d09p-33);  // 1/13!
+  q = fma_k(q, r, 0x1.1eed8eff8d898p-29);  // 1/12!
+  q = fma_k(q, r, 0x1.ae64567f544e4p-26);  // 1/11!
+  q = fma_k(q, r, 0x1.27e4fb7789f5cp-22);  // 1/10!
+  q = fma_k(q, r, 0x1.71de3a556c734p-19);  // 1/9!
+  q = fma_k(q, r, 0x1.a01a01a01a01ap-16);  // 1/8!
+  q = fma_k(q, r, 0x1.a01a01a01a01ap-13);  // 1/7!
+  q = fma_k(q, r, 0x1.6c16c16c16c17p-10);  // 1/6!
+  q = fma_k(q, r, 0x1.1111111111111p-7);   // 1/5!
+  q = fma_k(q, r, 0x1.5555555555555p-5);   // 1/4!
+  q = fma_k(q, r, 0x1.5555555555555p-3);   // 1/3!
+  q = fma_k(q, r, 0x1.0000000000000p-1);   // 1/2!
   q = (r * r) * q;
   const double one_r = 1.0 + r;
   const double one_e = (1.0 - one_r) + r;  // |r| < 1
