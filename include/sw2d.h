@@ -238,7 +238,9 @@ int sw2d_batch_detach(sw2d_batch *b, int id, sw2d_ctx **ctx_out);
  * as a scenario, with its state, clock, forcing and accumulators as they
  * are; the batch owns it from then on.  It joins the work-list at the next
  * sw2d_batch_step: nothing waits for the batch stream, so a context can be
- * created and given its forcing while the batch runs. */
+ * created and given its forcing while the batch runs.  *id_out is the
+ * lowest id that no scenario holds: ids freed by sw2d_batch_detach or
+ * sw2d_batch_remove are given out again. */
 int sw2d_batch_attach(sw2d_batch *b, sw2d_ctx *ctx, int *id_out);
 /* Segment length (rows) of one scenario's work items: a multiple of 4 in
  * [4, 4096]; 0 = the autotuned choice for its size bucket, or before any

@@ -299,12 +299,24 @@ int sw2d_batch_attach(sw2d_batch *b, sw2d_ctx *c, int *id_out) {
   // the batch's device lists are rewritten only by the next rebuild, which
   // waits for the batch stream: nothing here touches them
   c->owner = b->stream;
-  b->scen.push_back(c);
-  b->share_of.push_back(-1);
-  b->rows_override.push_back(0);
-  b->parked.push_back(0);
+  // the lowest id freed by a detach or remove is taken again (no live
+  // scenario can share the statics of a freed id: both calls refuse that),
+  // so a batch fed by attach and detach keeps a table as long as its peak
+  // number of scenarios
+  size_t id = 0;
+  while (id < b->scen.size() && b->scen[id]) id++;
+  if (id == b->scen.size()) {
+    b->scen.push_back(nullptr);
+    b->share_of.push_back(-1);
+    b->rows_override.push_back(0);
+    b->parked.push_back(0);
+  }
+  b->scen[id] = c;
+  b->share_of[id] = -1;
+  b->rows_override[id] = 0;
+  b->parked[id] = 0;
   b->dirty = true;
-  *id_out = (int)b->scen.size() - 1;
+  *id_out = (int)id;
   return SW2D_OK;
 }
 

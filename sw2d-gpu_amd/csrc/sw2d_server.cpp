@@ -167,21 +167,22 @@ struct sw2d_server {
     std::vector<int64_t> iters(std::max(c.n_added, 1));
     int rc = sw2d_batch_status(c.batch, done.data(), iters.data(), c.n_added, &n);
     if (rc) return rc;
+    // after a failed detach the rest stay in the batch and in `running`, so
+    // the class's lists still agree
     std::vector<int> still;
     for (int q : c.running) {
       Request &r = req[q];
-      if (!done[r.scen]) {
+      sw2d_ctx *ctx = nullptr;
+      if (rc || !done[r.scen] || (rc = sw2d_batch_detach(c.batch, r.scen, &ctx))) {
         still.push_back(q);
         continue;
       }
-      sw2d_ctx *ctx = nullptr;
-      if ((rc = sw2d_batch_detach(c.batch, r.scen, &ctx))) return rc;
       r.iterations = iters[r.scen];
       r.scen = -1;
       out.emplace_back(q, ctx);
     }
     c.running.swap(still);
-    return SW2D_OK;
+    return rc;
   }
 
   // Reads the final fields and statistics of detached scenarios into host

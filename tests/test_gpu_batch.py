@@ -227,7 +227,8 @@ def test_attach_a_stepped_context():
     """A context stepped on its own, then attached to a running batch with
     sw2d_batch_attach, continues from its state: after more batch steps it
     equals a run that was alone throughout, bitwise; a context already in a
-    batch is refused."""
+    batch is refused; detached and attached again, it takes its old id and
+    still matches."""
     say("case: attach a context stepped alone to a running batch")
     kw = dict(dt=0.2, manning=0.03, cfl=0.45, precision=sw2d.FP32, forcing_mode=sw2d.FORCING_STEP)
     b = sw2d.Batch(sw2d.FP32)
@@ -253,3 +254,14 @@ def test_attach_a_stepped_context():
     for f in (sw2d.F_H, sw2d.F_U, sw2d.F_V):
         assert np.array_equal(a.read(f), s.read(f)), f
     assert b.status()[1][first] == 134
+    # the id a detach frees is given to the next attach, and the scenario
+    # taken out and put back goes on from where it was
+    back = b.detach(sid)
+    assert b.attach(back) == sid
+    b.step(20)
+    a.step(20)
+    s = b.scenario(sid)
+    assert s.stats()["iterations"] == 135
+    for f in (sw2d.F_H, sw2d.F_U, sw2d.F_V):
+        assert np.array_equal(a.read(f), s.read(f)), f
+    assert b.status()[1][first] == 154
