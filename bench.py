@@ -182,10 +182,11 @@ def ensemble_bench(n_grids, chunk=100):
     return res
 
 
-def serving_bench(n_requests, rate, chunk=20):
+def serving_bench(n_requests, rate, chunk=20, active_cells=0):
     """Config 5: requests submitted at their Poisson arrival times (wall
-    clock) to the serving loop, which admits them by footprint, batches the
-    running ones and retires them; time-to-result = finish - submit."""
+    clock) to the serving loop, which admits them by footprint, activates
+    them least estimated work first, batches the active ones and retires
+    them; time-to-result = finish - submit."""
     import sw2d
     import workloads
     t0 = time.perf_counter()
@@ -196,7 +197,7 @@ def serving_bench(n_requests, rate, chunk=20):
     # device does not have free (runtime, other allocations) and 4 GiB
     free, total = sw2d.device_memory()
     budget = min(288 * 10 ** 9, free - (4 << 30))
-    srv = sw2d.Server(sw2d.FP32, mem_budget=budget)
+    srv = sw2d.Server(sw2d.FP32, mem_budget=budget, active_cells=active_cells)
     p = dict(w["params"])
     ids = [None] * len(reqs)
     nxt, finished = 0, set()
@@ -220,19 +221,26 @@ def serving_bench(n_requests, rate, chunk=20):
     info = [srv.info(i) for i in ids]
     srv.close()
     ttr = np.array([i["finish_s"] - i["submit_s"] for i in info])
-    queue = np.array([i["admit_s"] - i["submit_s"] for i in info])
-    service = np.array([i["finish_s"] - i["admit_s"] for i in info])
+    admission = np.array([i["admit_s"] - i["submit_s"] for i in info])
+    queue = np.array([i["start_s"] - i["submit_s"] for i in info])
+    service = np.array([i["finish_s"] - i["start_s"] for i in info])
     it = np.array([i["iterations"] for i in info])
     res = {"workload": w["desc"], "requests": len(reqs), "arrival_rate_per_s": rate,
            "wall_s": round(wall, 3), "scenarios_per_s": round(len(reqs) / wall, 3),
            "gcell_updates_per_s": round(float((cells * it).sum()) / wall / 1e9, 3),
            "time_to_result_s": {"p50": round(float(np.percentile(ttr, 50)), 3),
                                 "p99": round(float(np.percentile(ttr, 99)), 3)},
+           # queueing: submit -> first chunk on the device (admission by
+           # memory, then activation by least estimated work); service:
+           # first chunk -> result in host memory
            "queue_s": {"p50": round(float(np.percentile(queue, 50)), 3),
                        "p99": round(float(np.percentile(queue, 99)), 3)},
+           "admission_s": {"p50": round(float(np.percentile(admission, 50)), 3),
+                           "p99": round(float(np.percentile(admission, 99)), 3)},
            "service_s": {"p50": round(float(np.percentile(service, 50)), 3),
                          "p99": round(float(np.percentile(service, 99)), 3)},
            "memory_budget_bytes": budget, "device_memory_bytes": total,
+           "active_cells_target": active_cells or None,
            "cell_updates_total": float((cells * it).sum())}
     log("serving: %s" % res)
     return res
@@ -322,6 +330,8 @@ def main():
                     help="skip the FP32-vs-FP64 accuracy leg (config 2)")
     ap.add_argument("--ensemble-grids", type=int, default=64)
     ap.add_argument("--serving-requests", type=int, default=200)
+    ap.add_argument("--serving-active-cells", type=int, default=0,
+                    help="activation target of the serving loop (0: every admitted request active)")
     ap.add_argument("--sync-every", type=int, default=0,
                     help="run the warm-up iterations in chunks of this many, with a host "
                          "synchronisation after each (for counter runs under rocprofv3 --pmc, "
@@ -333,6 +343,8 @@ def main():
     ap.add_argument("--ensemble-only", action="store_true",
                     help="only run the batched-ensemble leg (config 4) and print its record "
                          "(for counter runs of the batch)")
+    ap.add_argument("--serving-only", action="store_true",
+                    help="only run the serving leg (config 5) and print its record")
     ap.add_argument("--stream-ceiling", action="store_true",
                     help="only time plain streaming kernels (torch's device copy and a two-input "
                          "add, 1 and 4 GiB per array) to give the HBM rate such kernels reach")
@@ -363,6 +375,10 @@ def main():
         return
     if args.ensemble_only:
         print(json.dumps(ensemble_bench(args.ensemble_grids)), flush=True)
+        return
+    if args.serving_only:
+        print(json.dumps(serving_bench(args.serving_requests, 50.0,
+                                       active_cells=args.serving_active_cells)), flush=True)
         return
     if args.copy_calibration:
         import torch
@@ -505,7 +521,7 @@ def main():
     if not args.no_ensemble:
         out["ensemble"] = ensemble_bench(args.ensemble_grids)
     if not args.no_serving:
-        out["serving"] = serving_bench(args.serving_requests, 50.0)
+        out["serving"] = serving_bench(args.serving_requests, 50.0, active_cells=args.serving_active_cells)
     print(json.dumps(out), flush=True)
 
 

@@ -272,18 +272,24 @@ int sw2d_batch_status(sw2d_batch *b, int *done, int64_t *iters, int cap, int *n_
  * Continuous batching of scenario requests on one device.  Requests queue
  * on submit (the server keeps host copies of their inputs) and are admitted
  * in arrival order while their device footprint (sw2d_footprint_bytes) fits
- * in the memory budget and a slot is free.  Admitted scenarios run in one
- * batch per size class (cells < 2^18, < 2^21, larger), each batch on its own
- * stream, so the classes advance concurrently and each at its own rate: a
- * small scenario is not held to the launch rate of the large ones.  Each
- * sw2d_server_advance waits until at least one class has finished its last
- * chunk of iterations, and for every such class retires the scenarios that
- * reached their t_end (final h, U, V and statistics go to host memory and
- * their device memory returns to the budget), admits queued requests of
- * that class, and enqueues its next chunk of iterations (n_iter, times 4
- * for the smallest class and 2 for the middle one) without waiting for it.
- * A scenario's result equals the same scenario run alone (sw2d_create +
- * sw2d_step), bitwise in h, U and V. */
+ * in the memory budget and a slot is free: admission creates the request's
+ * context (its device state).  Admitted requests become active at once,
+ * or, with a target set by sw2d_server_set_active_cells, least estimated
+ * work (cells x simulated seconds, divided by 1 + seconds waited / 60)
+ * first while the active ones hold fewer cells than the target (or none is
+ * active).  Active scenarios run in one batch per size class (cells < 2^18,
+ * < 2^21, larger), each batch on its own stream, so the classes advance
+ * concurrently and each at its own rate: a small scenario is not held to
+ * the launch rate of the large ones.  Each sw2d_server_advance waits until
+ * a class has finished its last chunk of iterations, a request can become
+ * active, or the head of the queue fits; then it takes the scenarios that
+ * reached their t_end out of the idle classes, activates waiting requests,
+ * enqueues each idle class's next chunk of iterations without waiting
+ * (n_iter times 128 for the smallest class, 32 for the middle one, 1 for
+ * the largest),
+ * admits queued requests, and reads the finished scenarios' final h, U, V
+ * and statistics into host memory.  A scenario's result equals the same
+ * scenario run alone (sw2d_create + sw2d_step), bitwise in h, U and V. */
 typedef struct sw2d_server sw2d_server;
 #define SW2D_REQ_QUEUED 0
 #define SW2D_REQ_RUNNING 1
@@ -295,6 +301,7 @@ typedef struct {
   double admit_s;          /* -1 until admitted                            */
   double finish_s;         /* -1 until retired                             */
   int64_t iterations;      /* committed iterations (when retired)          */
+  double start_s;          /* -1 until active (its first chunk of steps)   */
 } sw2d_request_info;
 /* mem_budget: device bytes the admitted scenarios may hold; max_slots:
  * concurrent scenarios (0 = no limit). */
@@ -305,9 +312,14 @@ void sw2d_server_destroy(sw2d_server *s);
  * Fails with SW2D_E_MEMORY when the request alone exceeds the budget. */
 int sw2d_server_submit(sw2d_server *s, const sw2d_grid *grid, const sw2d_params *params,
                        const double *rain_mm, int n_rain, double dtrain, int *req_id);
-/* Retire, admit and step the idle size classes (see above); waits only
- * until one class is idle.  Optional outputs: requests running and queued
- * after the call. */
+/* Activation target in cells: admitted requests wait until the active ones
+ * hold fewer cells, then start least estimated work first (shorter median
+ * time-to-result, at some cost in throughput); 0 (the default): every
+ * admitted request is active. */
+int sw2d_server_set_active_cells(sw2d_server *s, int64_t cells);
+/* Retire, activate, step and admit (see above).  Optional outputs after
+ * the call: requests admitted and not finished (active or waiting for the
+ * device), and requests queued for admission. */
 int sw2d_server_advance(sw2d_server *s, int64_t n_iter, int *n_running, int *n_queued);
 int sw2d_server_info(sw2d_server *s, int req_id, sw2d_request_info *info);
 /* Final field (SW2D_FIELD_H, _U or _V) or statistics of a retired request. */

@@ -6,24 +6,27 @@
 // Requests wait in a FIFO queue with host copies of their inputs.  Admission
 // takes the head of the queue while the admitted scenarios' device
 // footprint stays within the budget and a slot is free (strict arrival
-// order: a request that does not fit blocks those behind it).  Each size
-// class is a sw2d_batch with its own stream: a chunk of iterations of one
-// class is enqueued without waiting, so the classes' kernels run
-// concurrently and a class of small grids advances at its own launch rate
-// instead of the large grids'.  When a class's stream is idle its finished
-// scenarios leave the batch (sw2d_batch_detach) and its next chunk is
-// enqueued at once; then, while the chunks run, queued requests are
-// admitted (each context created on its own stream and attached to its
-// class's batch, joining the work-list at that class's next chunk) and the
-// finished scenarios' final fields and statistics are read into host
-// memory.  A retired scenario's context is not destroyed at once: freeing
-// device memory waits for the whole device, which would stall the other
-// classes' work, so retired contexts are kept (their memory still charged
-// to the budget) and destroyed together when an admission needs the
+// order: a request that does not fit blocks those behind it); it creates the
+// request's context on its own stream.  Admitted requests become active (join
+// their size class's batch with sw2d_batch_attach) at once, or, under an
+// activation target, least estimated work first while the active ones hold
+// fewer cells than the target.  Each size class is a sw2d_batch with its own
+// stream: a chunk of iterations of one class is enqueued without waiting,
+// so the classes' kernels run concurrently and a class of small grids
+// advances at its own launch rate instead of the large grids'.  When a
+// class's stream is idle its finished scenarios leave the batch
+// (sw2d_batch_detach), waiting requests are activated and the class's next
+// chunk is enqueued at once; then, while the chunks run, queued requests are
+// admitted and the finished scenarios' final fields and statistics are read
+// into host memory.  A retired scenario's context is not destroyed at once:
+// freeing device memory waits for the whole device, which would stall the
+// other classes' work, so retired contexts are kept (their memory still
+// charged to the budget) and destroyed together when an admission needs the
 // memory, or with the server.
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <limits>
 #include <deque>
 #include <string>
 #include <thread>
@@ -50,19 +53,27 @@ struct Request {
   std::vector<double> rain;
   double dtrain = 0;
   size_t footprint = 0;
-  double submit_s = 0, admit_s = -1, finish_s = -1;
+  double submit_s = 0, admit_s = -1, start_s = -1, finish_s = -1;
   int64_t iterations = 0;
   int cls = 0;                   // size class
-  int scen = -1;                 // scenario id in its class's batch while running
+  long long cells = 0;
+  double work = 0;               // estimated work: cells x simulated seconds
+  sw2d_ctx *ctx = nullptr;       // admitted, waiting for the device: its context
+  int scen = -1;                 // scenario id in its class's batch while active
   std::vector<double> H, U, V;   // final fields once retired
   sw2d_stats stats{};
 };
 
-// Size classes by cells; a chunk of iterations is n_iter times the class's
-// multiplier (small grids take many cheap iterations per host round trip).
+// Size classes by cells; a class's chunk is n_iter iterations times its
+// multiplier.  The largest class's short chunks set the period of the host
+// loop, in which admissions and read-backs happen; a class idles from the
+// end of its chunk until the loop comes back to it, so the classes of small
+// grids, whose iterations are short, take long chunks (on the 200-request
+// serving stream, multipliers 128 and 32 against 4 and 2 halved the median
+// time-to-result at the same throughput and p99).
 constexpr int N_CLASS = 3;
 constexpr long long CLASS_CELLS[N_CLASS - 1] = {1LL << 18, 1LL << 21};
-constexpr int CLASS_MULT[N_CLASS] = {4, 2, 1};
+constexpr int CLASS_MULT[N_CLASS] = {128, 32, 1};
 int class_of(int rows, int cols) {
   const long long c = (long long)rows * cols;
   int k = 0;
@@ -70,9 +81,17 @@ int class_of(int rows, int cols) {
   return k;
 }
 
+// Activation: admitted requests join their class's batch, least estimated
+// work first, while the active scenarios hold fewer cells than the server's
+// target (sw2d_server_set_active_cells; by default every admitted request is
+// active); a request's estimated work is divided by 1 + (seconds waited) /
+// ACTIVE_AGING_S, so that a long request waiting behind a stream of short
+// ones still starts.
+constexpr double ACTIVE_AGING_S = 60.0;
+
 struct Class {
   sw2d_batch *batch = nullptr;
-  std::vector<int> running;  // request ids
+  std::vector<int> running;  // active request ids
   int n_added = 0;           // scenario ids the batch has handed out (0 .. n_added-1)
 };
 
@@ -85,6 +104,9 @@ struct sw2d_server {
   Class cls[N_CLASS];
   std::vector<Request> req;
   std::deque<int> queue;
+  std::vector<int> waiting;      // admitted, not yet active (contexts created)
+  long long active_target = std::numeric_limits<long long>::max();
+  long long active_cells = 0;
   size_t used = 0;
   // retired scenarios not yet destroyed, and their footprints
   std::vector<std::pair<sw2d_ctx *, size_t>> retired;
@@ -103,25 +125,28 @@ struct sw2d_server {
   double now() const {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
-  int n_running() const {
+  int n_active() const {
     int n = 0;
     for (const Class &c : cls) n += (int)c.running.size();
     return n;
   }
+  // admitted requests: active, or waiting for the device
+  int n_running() const { return n_active() + (int)waiting.size(); }
+  bool can_activate() const {
+    return !waiting.empty() && (active_cells < active_target || n_active() == 0);
+  }
 
   // Admit from the head of the queue (strict arrival order) while the
-  // footprint fits the budget and a slot is free.  A request's context is
-  // created on its own stream and given its rain before it joins its class's
-  // batch (sw2d_batch_attach waits for no batch stream): the uploads run
-  // while the classes' chunks do, and the scenario enters the work-list at
-  // its class's next chunk.
+  // footprint fits the budget and a slot is free: the request's context is
+  // created on its own stream and given its rain (the uploads run while the
+  // classes' chunks do); it waits there until activate() puts it in its
+  // class's batch.
   int admit() {
     while (!queue.empty()) {
       Request &r = req[queue.front()];
       if (max_slots > 0 && n_running() >= max_slots) break;
       if (used + r.footprint > budget && !retired.empty()) free_retired();
       if (used + r.footprint > budget) break;
-      Class &c = cls[r.cls];
       sw2d_grid g;
       g.rows = r.rows;
       g.cols = r.cols;
@@ -136,23 +161,54 @@ struct sw2d_server {
       if (rc) return rc;
       if (!r.rain.empty())
         rc = sw2d_set_rain(ctx, r.rain.data(), (int)r.rain.size(), r.dtrain, nullptr, 0);
-      int id = -1;
-      if (!rc) rc = sw2d_batch_attach(c.batch, ctx, &id);
       if (rc) {
         // not admitted: the request stays at the head of the queue
         sw2d_destroy(ctx);
         return rc;
       }
-      c.n_added = std::max(c.n_added, id + 1);
-      r.scen = id;
+      r.ctx = ctx;
       r.state = SW2D_REQ_RUNNING;
       r.admit_s = now();
       std::vector<double>().swap(r.z);
       std::vector<double>().swap(r.h0);
       std::vector<double>().swap(r.n);
       used += r.footprint;
-      c.running.push_back(queue.front());
+      waiting.push_back(queue.front());
       queue.pop_front();
+    }
+    return SW2D_OK;
+  }
+
+  // Puts waiting requests in their classes' batches, least (aged) estimated
+  // work first, while the active cells are below the target or nothing is
+  // active.  sw2d_batch_attach waits for no batch stream: a request joins
+  // the work-list at its class's next chunk.
+  int activate() {
+    const double t = now();
+    while (can_activate()) {
+      size_t best = 0;
+      double best_key = 0;
+      for (size_t j = 0; j < waiting.size(); j++) {
+        const Request &r = req[waiting[j]];
+        const double key = r.work / (1.0 + (t - r.submit_s) / ACTIVE_AGING_S);
+        if (j == 0 || key < best_key) {
+          best = j;
+          best_key = key;
+        }
+      }
+      const int q = waiting[best];
+      Request &r = req[q];
+      Class &c = cls[r.cls];
+      int id = -1;
+      const int rc = sw2d_batch_attach(c.batch, r.ctx, &id);
+      if (rc) return rc;  // the request keeps waiting with its context
+      r.ctx = nullptr;
+      r.scen = id;
+      r.start_s = t;
+      c.n_added = std::max(c.n_added, id + 1);
+      c.running.push_back(q);
+      active_cells += r.cells;
+      waiting.erase(waiting.begin() + (long)best);
     }
     return SW2D_OK;
   }
@@ -179,6 +235,7 @@ struct sw2d_server {
       }
       r.iterations = iters[r.scen];
       r.scen = -1;
+      active_cells -= r.cells;
       out.emplace_back(q, ctx);
     }
     c.running.swap(still);
@@ -210,8 +267,9 @@ struct sw2d_server {
 
   int advance(int64_t n_iter) {
     bool idle[N_CLASS];
-    // wait until a class with running scenarios has finished its last
-    // chunk, or the request at the head of the queue fits (or nothing runs)
+    // wait until a class with active scenarios has finished its last
+    // chunk, a waiting request can become active, or the request at the
+    // head of the queue fits (or nothing runs)
     for (;;) {
       bool ready = false, busy = false;
       for (int k = 0; k < N_CLASS; k++) {
@@ -225,6 +283,7 @@ struct sw2d_server {
       if (!queue.empty() && (max_slots == 0 || n_running() < max_slots) &&
           (used + req[queue.front()].footprint <= budget || !retired.empty()))
         ready = true;
+      if (can_activate()) ready = true;
       if (ready || !busy) break;
       std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
@@ -238,14 +297,18 @@ struct sw2d_server {
           return rc;
         }
       }
-    // 2. their next chunks go to the device
-    int rc = SW2D_OK;
+    // 2. waiting requests take the freed room, and the idle classes' next
+    // chunks go to the device
+    int rc = activate();
     for (int k = 0; k < N_CLASS && !rc; k++) {
       Class &c = cls[k];
       if (idle[k] && !c.running.empty() && n_iter > 0) rc = sw2d_batch_step(c.batch, n_iter * CLASS_MULT[k]);
     }
-    // 3. while they run: admissions and the finished scenarios' read-back
+    // 3. while they run: admissions (a request admitted now may become
+    // active at once, for its class's next chunk) and the finished
+    // scenarios' read-back
     if (!rc) rc = admit();
+    if (!rc) rc = activate();
     const int rf = finish(finished);
     return rc ? rc : rf;
   }
@@ -272,6 +335,7 @@ int sw2d_server_create(int precision, size_t mem_budget, int max_slots, sw2d_ser
 void sw2d_server_destroy(sw2d_server *s) {
   if (!s) return;
   s->free_retired();
+  for (int q : s->waiting) sw2d_destroy(s->req[q].ctx);
   for (Class &c : s->cls)
     if (c.batch) sw2d_batch_destroy(c.batch);
   delete s;
@@ -300,11 +364,19 @@ int sw2d_server_submit(sw2d_server *s, const sw2d_grid *grid, const sw2d_params 
   r.dtrain = dtrain;
   r.footprint = sw2d_footprint_bytes(r.rows, r.cols, s->prec, r.has_n);
   r.cls = class_of(r.rows, r.cols);
+  r.cells = (long long)r.rows * r.cols;
+  r.work = (double)r.cells * std::max(0.0, params->t_end - params->t0);
   if (r.footprint > s->budget) return srv_fail(SW2D_E_MEMORY, "request footprint exceeds the memory budget");
   r.submit_s = s->now();
   s->req.push_back(std::move(r));
   *req_id = (int)s->req.size() - 1;
   s->queue.push_back(*req_id);
+  return SW2D_OK;
+}
+
+int sw2d_server_set_active_cells(sw2d_server *s, int64_t cells) {
+  if (!s || cells < 0) return srv_fail(SW2D_E_INVALID, "active cells");
+  s->active_target = cells > 0 ? (long long)cells : std::numeric_limits<long long>::max();
   return SW2D_OK;
 }
 
@@ -329,6 +401,7 @@ int sw2d_server_info(sw2d_server *s, int req_id, sw2d_request_info *info) {
   info->cols = r->cols;
   info->submit_s = r->submit_s;
   info->admit_s = r->admit_s;
+  info->start_s = r->start_s;
   info->finish_s = r->finish_s;
   info->iterations = r->iterations;
   return SW2D_OK;

@@ -34,7 +34,8 @@ EXPORTS = [
     "sw2d_batch_add", "sw2d_batch_scenario", "sw2d_batch_touch", "sw2d_batch_remove", "sw2d_batch_detach", "sw2d_batch_attach",
     "sw2d_batch_set_tile_rows", "sw2d_batch_autotune", "sw2d_batch_tile_rows",
     "sw2d_batch_step", "sw2d_batch_sync", "sw2d_batch_status", "sw2d_batch_idle", "sw2d_device_memory",
-    "sw2d_server_create", "sw2d_server_destroy", "sw2d_server_submit", "sw2d_server_advance",
+    "sw2d_server_create", "sw2d_server_destroy", "sw2d_server_submit", "sw2d_server_set_active_cells",
+    "sw2d_server_advance",
     "sw2d_server_info", "sw2d_server_read", "sw2d_server_stats", "sw2d_server_release",
 ]
 
@@ -67,7 +68,7 @@ class Stats(C.Structure):
 class RequestInfo(C.Structure):
     _fields_ = [("state", C.c_int), ("rows", C.c_int), ("cols", C.c_int),
                 ("submit_s", C.c_double), ("admit_s", C.c_double), ("finish_s", C.c_double),
-                ("iterations", C.c_int64)]
+                ("iterations", C.c_int64), ("start_s", C.c_double)]
 
 
 class CaseInfo(C.Structure):
@@ -160,6 +161,7 @@ def lib():
         L.sw2d_server_submit.argtypes = [vp, C.POINTER(Grid), C.POINTER(Params), dp, C.c_int,
                                          C.c_double, ip]
         L.sw2d_server_advance.argtypes = [vp, C.c_int64, ip, ip]
+        L.sw2d_server_set_active_cells.argtypes = [vp, C.c_int64]
         L.sw2d_server_info.argtypes = [vp, C.c_int, C.POINTER(RequestInfo)]
         L.sw2d_server_read.argtypes = [vp, C.c_int, C.c_int, dp]
         L.sw2d_server_stats.argtypes = [vp, C.c_int, C.POINTER(Stats)]
@@ -536,12 +538,18 @@ class Batch:
 class Server:
     """Serving loop with continuous batching (sw2d_server_* in include/sw2d.h)."""
 
-    def __init__(self, precision=FP32, mem_budget=1 << 30, max_slots=0):
+    def __init__(self, precision=FP32, mem_budget=1 << 30, max_slots=0, active_cells=0):
         h = C.c_void_p()
         check(lib().sw2d_server_create(precision, int(mem_budget), int(max_slots), C.byref(h)))
         self.h_ = h
         self.precision = precision
         self.shapes = []
+        if active_cells:
+            self.set_active_cells(active_cells)
+
+    def set_active_cells(self, cells):
+        """Activation target (0: every admitted request active)."""
+        check(lib().sw2d_server_set_active_cells(self.h_, int(cells)))
 
     def close(self):
         if getattr(self, "h_", None):
@@ -572,7 +580,8 @@ class Server:
         return out.value
 
     def advance(self, n_iter):
-        """Admit, step n_iter iterations, retire; returns (running, queued)."""
+        """Retire, activate, step n_iter iterations, admit; returns (admitted
+        and not finished, queued for admission)."""
         nr, nq = C.c_int(), C.c_int()
         check(lib().sw2d_server_advance(self.h_, int(n_iter), C.byref(nr), C.byref(nq)))
         return nr.value, nq.value

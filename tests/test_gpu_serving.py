@@ -1,7 +1,8 @@
 """Serving loop (sw2d_server_*): continuous batching of requests that
 arrive while others run.  Each result must be the one the same scenario
 gives alone (sw2d_create + sw2d_step), bitwise in h, U and V; admission
-follows arrival order within the memory budget."""
+follows arrival order within the memory budget, activation least estimated
+work first."""
 import os
 import sys
 
@@ -58,7 +59,7 @@ def test_served_results_equal_alone(precision):
     for k, rid in enumerate(ids):
         info = srv.info(rid)
         assert info["state"] == sw2d.REQ_DONE
-        assert info["admit_s"] >= info["submit_s"] and info["finish_s"] >= info["admit_s"]
+        assert info["submit_s"] <= info["admit_s"] <= info["start_s"] <= info["finish_s"]
         z, h0, n, rain, params = request(k)
         a = run_alone(precision, z, h0, n, rain, params)
         sa, sb = a.stats(), srv.stats(rid)
@@ -95,6 +96,37 @@ def test_admission_follows_arrival_order_within_budget():
     assert admits == sorted(admits)
     for a, b in zip(infos, infos[1:]):  # each admitted only after the previous finished
         assert b["admit_s"] >= a["finish_s"]
+    srv.close()
+
+
+def test_activation_least_work_first():
+    """With room for one active request (an activation target of 1 cell),
+    requests admitted together start least estimated work first (cells x
+    simulated seconds), whatever their arrival order, and each still gives
+    the result of its alone run."""
+    say("case: activation order by estimated work, one active at a time")
+    srv = sw2d.Server(sw2d.FP32, mem_budget=1 << 30, active_cells=1)
+    ks = [3, 0, 5, 1]  # arrival order
+    reqs = {k: request(k) for k in ks}
+    ids = {k: srv.submit(z, h0, 5.0, n_cell=n, rain=rain, **p) for k, (z, h0, n, rain, p) in reqs.items()}
+    peak = 0
+    for _ in range(4000):
+        running, queued = srv.advance(25)
+        peak = max(peak, running)
+        if running == 0 and queued == 0:
+            break
+    assert peak == 4  # all four admitted (resident) together
+    infos = {k: srv.info(ids[k]) for k in ks}
+    assert all(i["state"] == sw2d.REQ_DONE for i in infos.values())
+    work = {k: reqs[k][0].size * reqs[k][4]["t_end"] for k in ks}
+    by_start = sorted(ks, key=lambda k: infos[k]["start_s"])
+    assert by_start == sorted(ks, key=lambda k: work[k]), (by_start, work)
+    for k in ks:
+        assert infos[k]["submit_s"] <= infos[k]["admit_s"] <= infos[k]["start_s"] <= infos[k]["finish_s"]
+    z, h0, n, rain, p = reqs[3]
+    a = run_alone(sw2d.FP32, z, h0, n, rain, p)
+    for f in (sw2d.F_H, sw2d.F_U, sw2d.F_V):
+        assert np.array_equal(a.read(f), srv.read(ids[3], f)), f
     srv.close()
 
 
