@@ -203,18 +203,29 @@ def serving_bench(n_requests, rate, chunk=20, active_cells=0):
     nxt, finished = 0, set()
     cells = np.array([r["z"].size for r in reqs], dtype=np.int64)
     start = time.perf_counter()
+    pending = []
+    next_log = 10.0
     while len(finished) < len(reqs):
         now = time.perf_counter() - start
         while nxt < len(reqs) and reqs[nxt]["arrival"] <= now:
             r = reqs[nxt]
             ids[nxt] = srv.submit(r["z"], np.zeros_like(r["z"]), w["dx"], n_cell=r["n_cell"],
                                   rain=r["rain"], dtrain=w["dtrain"], t_end=r["duration"], **p)
+            pending.append(nxt)
             nxt += 1
         running, queued = srv.advance(chunk if nxt > len(finished) else 0)
-        for k in range(nxt):
-            if k not in finished and srv.info(ids[k])["state"] == sw2d.REQ_DONE:
+        still = []
+        for k in pending:
+            if srv.info(ids[k])["state"] == sw2d.REQ_DONE:
                 finished.add(k)
                 srv.release(ids[k])
+            else:
+                still.append(k)
+        pending = still
+        if now >= next_log:
+            log("serving: %.0f s, %d of %d finished, %d admitted, %d queued"
+                % (now, len(finished), len(reqs), running, queued))
+            next_log += 10.0
         if running == 0 and queued == 0 and nxt < len(reqs):
             time.sleep(max(0.0, reqs[nxt]["arrival"] - (time.perf_counter() - start)))
     wall = time.perf_counter() - start
@@ -268,7 +279,10 @@ def accuracy_config2(size=1024):
     cap = 200000
     a = workloads.make_sim(w, precision=sw2d.FP64)
     a.record_dt(cap)
+    a.sync()
+    t64 = time.perf_counter()
     run(a)
+    w64 = time.perf_counter() - t64
     D = a.dt_history(cap)
 
     def replay(prec, z=None):
@@ -277,8 +291,13 @@ def accuracy_config2(size=1024):
         return run(s)
 
     rng = np.random.default_rng(1002)
+    s32 = workloads.make_sim(w, precision=sw2d.FP32)
+    s32.sync()
+    t32 = time.perf_counter()
+    run(s32)
+    w32 = time.perf_counter() - t32
     runs = {"fp32_same_steps": replay(sw2d.FP32),
-            "fp32_own_cfl": run(workloads.make_sim(w, precision=sw2d.FP32)),
+            "fp32_own_cfl": s32,
             "fp64_bed_1e-7": replay(sw2d.FP64, z=w["z"] + rng.choice([-1e-7, 1e-7], w["z"].shape))}
     ha = a.read(sw2d.F_H)
     sa = a.stats()
@@ -287,7 +306,12 @@ def accuracy_config2(size=1024):
         inflow = st["rain_in"] + st["source_in"] - st["evap_out"] - st["outflow"] + st["clamp_gain"]
         return abs(st["volume"] - inflow) / max(st["volume"], 1e-30)
 
+    cells = w["z"].size
     res = {"workload": w["desc"], "iterations_fp64": int(sa["iterations"]),
+           # whole 3600 s runs, wall clock (host synchronisation every 256
+           # iterations to test the end time)
+           "gpu_fp64_gcell_updates_per_s": round(cells * sa["iterations"] / w64 / 1e9, 3),
+           "gpu_fp32_gcell_updates_per_s": round(cells * s32.stats()["iterations"] / w32 / 1e9, 3),
            "volume_fp64_m3": round(sa["volume"], 3), "fp64_mass_balance": closure(sa),
            "fp64_terms_m3": {k2: round(sa[k2], 4) for k2 in
                              ("volume", "rain_in", "outflow", "clamp_gain", "evap_out", "source_in")}}
@@ -305,8 +329,52 @@ def accuracy_config2(size=1024):
                                ("volume", "rain_in", "outflow", "clamp_gain", "evap_out", "source_in")}}
         s.close()
     a.close()
+    res["cpu_baseline"] = cpu_sample(w, 200)
     res["wall_s"] = round(time.perf_counter() - t0, 2)
     log("accuracy (config 2): %s" % res)
+    return res
+
+
+def cpu_sample(w, n_iter):
+    """CPU baseline of configurations 1 and 2: the FP64 OpenMP oracle on the
+    workload from its initial state, n_iter iterations, timed."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as O
+    p = w["params"]
+    o = O.OracleSim(w["z"], np.ones(w["z"].shape, np.uint8), w["h0"], w["dx"], p["dt"], p["manning"],
+                    cfl=p.get("cfl", 0.0), edge_open=p.get("edge_open", 0),
+                    forcing_mode=p.get("forcing_mode", 0), n_cell=w["n_cell"])
+    if w["rain"] is not None:
+        o.set_rain(*w["rain"])
+    t0 = time.perf_counter()
+    o.step(n_iter)
+    dt = time.perf_counter() - t0
+    return {"value": round(w["z"].size * n_iter / dt / 1e9, 4), "unit": "Gcell-updates/s",
+               "iterations": n_iter, "wall_s": round(dt, 3),
+               "threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))}
+
+
+def config1_bench():
+    """Configuration 1 (the 200 x 100 FP64 dam break, 1000 iterations): the
+    whole run on the GPU (HIP-graph replay, one host synchronisation) and,
+    as its CPU baseline, in the FP64 OpenMP oracle."""
+    import sw2d
+    import workloads
+    w = workloads.config1()
+    s = workloads.make_sim(w)
+    s.step(1)  # first launch (graph capture), not timed; the run restarts below
+    s.close()
+    s = workloads.make_sim(w)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(w["iterations"])
+    s.sync()
+    wall = time.perf_counter() - t0
+    s.close()
+    cells = w["z"].size
+    res = {"workload": w["desc"], "gpu_fp64_gcell_updates_per_s": round(cells * w["iterations"] / wall / 1e9, 4),
+           "gpu_wall_s": round(wall, 4), "cpu_baseline": cpu_sample(w, w["iterations"])}
+    log("config 1: %s" % res)
     return res
 
 
@@ -518,6 +586,7 @@ def main():
         log("cpu baseline: %s" % out["cpu_baseline"])
     if not args.no_accuracy:
         out["accuracy"] = accuracy_config2()
+        out["config1"] = config1_bench()
     if not args.no_ensemble:
         out["ensemble"] = ensemble_bench(args.ensemble_grids)
     if not args.no_serving:
