@@ -13,7 +13,7 @@ namespace sw2d {
 // segments save work, but the launch runs in rounds of `slots` concurrent
 // wavefronts (CUs x 4 SIMDs x waves per SIMD) and a partly filled last round
 // idles the chip.  The default takes the longest segments that still give
-// about `rounds` full rounds, never fewer than 16 rows:
+// about `rounds` full rounds, of at least 16 rows on grids large enough:
 //  * FP32 (4,096 slots), three rounds: whole 8192^2 runs, interleaved on one
 //    MI355X, took 100 rows (2.94 rounds) to 115.0 Gcell/s, 76 rows (3.9
 //    rounds) to 114.2 and 152 rows (1.94 rounds) to 112.0; 296 rows (one
@@ -26,8 +26,15 @@ namespace sw2d {
 //    halo reuse in L2.
 // On larger grids the number of rounds grows until the segments are no
 // longer than `max_rows` (160 FP32, 64 FP64), the longest measured to pay.
-// SW2D_SEG_ROWS in the environment overrides it (a multiple of 4 in
-// [4, 4096]).
+// A grid too small for `rounds` rounds of 16-row segments leaves most of the
+// chip idle: there an iteration lasts about one wavefront's walk, and the
+// default is the shortest segment (at least 4 rows) whose items fit in one
+// round.  Per iteration, 16-row segments against this rule: the 200 x 100
+// FP64 dam break 59 against 27 us (4 rows), a 1024^2 FP32 catchment 36.7
+// against 31.0 us (8 rows; 4 rows, 1.2 rounds, 38.9), the same in FP64 90
+// against 75 us (8 rows), a 256^2 FP32 catchment 28.7 against 18.2 us (4
+// rows).  SW2D_SEG_ROWS in the environment overrides the default (a
+// multiple of 4 in [4, 4096]).
 inline int default_seg_rows(int rows, int strips, long long slots, int rounds, int max_rows) {
   if (const char *e = getenv("SW2D_SEG_ROWS")) {
     const int v = atoi(e);
@@ -41,7 +48,13 @@ inline int default_seg_rows(int rows, int strips, long long slots, int rounds, i
     s = (s + 3) / 4 * 4;
     if (s <= max_rows) break;
   }
-  return (int)(s < 16 ? 16 : s > 4096 ? 4096 : s);
+  if (s < 16) {  // a small grid: one round of the shortest segments
+    const long long segs = slots / st;
+    s = segs > 0 ? (rows + segs - 1) / segs : rows;
+    s = (s + 3) / 4 * 4;
+    if (s < 4) s = 4;
+  }
+  return (int)(s > 4096 ? 4096 : s);
 }
 
 // Longest segment (a multiple of 4, at most 4096 rows) whose reach in the
@@ -57,7 +70,9 @@ inline int reach_rows_cap(long long row_bytes) {
 // Segment rows of the scenarios of a batch whose live set holds
 // `strip_rows` strip rows in all (sum over scenarios of strips x rows): the
 // height that gives the whole set about twelve rounds of wave slots, at
-// least 16 and at most `max_rows`.  A batch mixes segments of different
+// least 16 and at most `max_rows`; a live set too small for that takes the
+// shortest segments (at least 4 rows) that fit in one round, as a small
+// single grid does (default_seg_rows).  A batch mixes segments of different
 // cost (wet and dry grids, short last segments), so it needs more rounds
 // than one uniform grid to keep the end of a launch busy.  On the 256
 // scenarios of 64 grids (sides 259-1955, 3.36 M strip rows, FP32), whole
@@ -67,10 +82,15 @@ inline int reach_rows_cap(long long row_bytes) {
 // SW2D_SEG_ROWS is set (the override then applies to every scenario).
 inline int batch_seg_rows(long long strip_rows, long long slots, int max_rows) {
   if (getenv("SW2D_SEG_ROWS")) return 0;
-  const long long per = 12 * (slots > 0 ? slots : 1);
-  long long s = (strip_rows + per - 1) / per;
+  const long long sl = slots > 0 ? slots : 1;
+  long long s = (strip_rows + 12 * sl - 1) / (12 * sl);
   s = (s + 3) / 4 * 4;
-  return (int)(s < 16 ? 16 : s > max_rows ? max_rows : s);
+  if (s < 16) {
+    s = (strip_rows + sl - 1) / sl;
+    s = (s + 3) / 4 * 4;
+    if (s < 4) s = 4;
+  }
+  return (int)(s > max_rows ? max_rows : s);
 }
 
 // concurrent single-wavefront workgroups of the step kernel on the current
