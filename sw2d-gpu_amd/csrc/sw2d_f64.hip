@@ -7,7 +7,7 @@
 namespace sw2d {
 
 hipError_t launch_control(Ctrl *const *ctrl, int n, int commit_only, hipStream_t s) {
-  hipLaunchKernelGGL(k_control, dim3((n + 63) / 64), dim3(64), 0, s, ctrl, n, commit_only);
+  hipLaunchKernelGGL(k_control, dim3(n), dim3(WAVE), 0, s, ctrl, n, commit_only);
   return hipGetLastError();
 }
 
@@ -17,7 +17,7 @@ hipError_t launch_status(Ctrl *const *ctrl, int n, int *done, long long *iters, 
 }
 
 hipError_t launch_iteration_f64(const IterLaunch &a, hipStream_t s) {
-  hipLaunchKernelGGL(k_control, dim3((a.n_ctrl + 63) / 64), dim3(64), 0, s, a.ctrl, a.n_ctrl, 0);
+  hipLaunchKernelGGL(k_control, dim3(a.n_ctrl), dim3(WAVE), 0, s, a.ctrl, a.n_ctrl, 0);
   const Grid<double> *g = static_cast<const Grid<double> *>(a.grids);
   if (a.ev) (void)hipEventRecord(a.ev[0], s);
   hipLaunchKernelGGL(k_step<double>, dim3(a.blocks), dim3(WAVE), 0, s, g, a.work, a.n_work);
