@@ -378,6 +378,34 @@ def config1_bench():
     return res
 
 
+def stream_5r3w(n=8192 * 8192):
+    """The step's bytes per cell in the most favourable pattern: five float
+    arrays read and three written per element, grid-stride over 8192^2
+    elements (libsw2d_probe.so, built with the library), in several launch
+    shapes, dword and float4 loads.  Rates count the 32 bytes per element."""
+    import ctypes as C
+    import torch
+    lib = C.CDLL(os.path.join(ROOT, "sw2d-gpu_amd", "libsw2d_probe.so"))
+    lib.sw2d_probe_stream5r3w.argtypes = [C.POINTER(C.c_void_p), C.c_longlong, C.c_int, C.c_int, C.c_int,
+                                          C.c_int, C.POINTER(C.c_double)]
+    arrs = [torch.rand(n, device="cuda") for _ in range(5)] + [torch.empty(n, device="cuda") for _ in range(3)]
+    ptrs = (C.c_void_p * 8)(*[a.data_ptr() for a in arrs])
+    torch.cuda.synchronize()
+    res = {}
+    for vec4 in (0, 1):
+        for blocks in (2048, 4096, 8192, 16384):
+            ms = C.c_double()
+            if lib.sw2d_probe_stream5r3w(ptrs, n, blocks, 256, vec4, 20, C.byref(ms)) != 0:
+                raise RuntimeError("stream probe failed")
+            res["%s_%d" % ("float4" if vec4 else "dword", blocks)] = round(32.0 * n / (ms.value * 1e-3) / 1e12, 3)
+    del arrs
+    torch.cuda.empty_cache()
+    best = max(res.values())
+    log("5-read / 3-write stream (TB/s at 32 B per element): %s" % res)
+    return {"stream_5r3w_TBps": res, "stream_5r3w_best_TBps": best,
+            "stream_5r3w_best_ms_per_8192sq": round(32.0 * n / (best * 1e12) * 1e3, 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -415,7 +443,9 @@ def main():
                     help="only run the serving leg (config 5) and print its record")
     ap.add_argument("--stream-ceiling", action="store_true",
                     help="only time plain streaming kernels (torch's device copy and a two-input "
-                         "add, 1 and 4 GiB per array) to give the HBM rate such kernels reach")
+                         "add, 1 and 4 GiB per array, and a five-read / three-write sweep over "
+                         "8192^2 floats, the step's bytes per cell) to give the HBM rate such "
+                         "kernels reach")
     args = ap.parse_args()
     if args.stream_ceiling:
         import torch
@@ -439,6 +469,7 @@ def main():
                 out["%s_%dGiB_TBps" % (name, gib)] = round(nbytes * 20 / (e0.elapsed_time(e1) * 1e-3) / 1e12, 3)
             del a, b, c
             torch.cuda.empty_cache()
+        out.update(stream_5r3w())
         print(json.dumps(out), flush=True)
         return
     if args.ensemble_only:
