@@ -35,6 +35,20 @@ def test_library_exports_every_declared_symbol():
     assert set(names) == set(sw2d.EXPORTS)
 
 
+def test_stream_probe_library_loads():
+    """The streaming-ceiling probe (bench.py --stream-ceiling) is built beside
+    the library and exports its entry point; with a bad argument it refuses
+    before touching the device."""
+    import ctypes as C
+    path = os.path.join(ROOT, "sw2d-gpu_amd", "libsw2d_probe.so")
+    L = C.CDLL(path)
+    f = L.sw2d_probe_stream5r3w
+    f.argtypes = [C.POINTER(C.c_void_p), C.c_longlong, C.c_int, C.c_int, C.c_int, C.c_int,
+                  C.POINTER(C.c_double)]
+    ms = C.c_double()
+    assert f(None, 1024, 1, 64, 0, 1, C.byref(ms)) == -1
+
+
 def test_case_reader_matches_restatement(dam_case):
     c = sw2d.Case(DAM_BREAK)
     I = c.info
