@@ -45,24 +45,26 @@ def traffic_per_launch(size, precision):
 
 
 def cpu_model():
-    """The host CPU's model name (lscpu) and the number of online CPUs."""
+    """The host CPU's model name (lscpu's "Model name", else /proc/cpuinfo)
+    and the number of online CPUs."""
+    import subprocess
     model = None
     try:
-        with open("/proc/cpuinfo") as f:
-            for ln in f:
-                if ln.startswith("model name"):
-                    model = ln.split(":", 1)[1].strip()
-                    break
-    except OSError:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for ln in out.splitlines():
+            if ln.startswith("Model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
         pass
-    if model is None:
+    if not model:
         try:
-            import subprocess
-            out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
-            for ln in out.splitlines():
-                if ln.startswith("Model name"):
-                    model = ln.split(":", 1)[1].strip()
-        except (OSError, subprocess.SubprocessError):
+            with open("/proc/cpuinfo") as f:
+                for ln in f:
+                    if ln.startswith("model name"):
+                        model = ln.split(":", 1)[1].strip()
+                        break
+        except OSError:
             pass
     return model, os.cpu_count()
 
