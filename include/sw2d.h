@@ -67,7 +67,8 @@ typedef struct {
 typedef struct {
   double dt;               /* half step; one iteration advances 2*dt       */
   double g;                /* 9.80 in the reference (L:809)                */
-  double th;               /* dry threshold, 1e-4 (L:1130); FP32: >= 1e-30 */
+  double th;               /* dry threshold, 1e-4 (L:1130); FP32: >= 1e-30,
+                              FP64: g * th >= 2^-760                      */
   double manning;          /* scalar n (L:799), used without a raster      */
   double INT, INF, LWL, EV_WL_min; /* runoff fractions (L:627-634)         */
   int evaporation_on;

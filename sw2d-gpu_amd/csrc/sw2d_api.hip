@@ -181,6 +181,10 @@ int sw2d_create(const sw2d_grid *grid, const sw2d_params *params, sw2d_ctx **out
   // normal float, so that such depths are too
   if (params->precision == SW2D_FP32 && !(params->th >= 1e-30))
     return fail(SW2D_E_INVALID, "th must be >= 1e-30 in the FP32 form");
+  // the FP64 step takes square roots of g h for depths h > th without the
+  // range handling of arguments below 2^-767
+  if (params->precision == SW2D_FP64 && !(params->g * params->th >= 0x1p-760))
+    return fail(SW2D_E_INVALID, "g * th must be >= 2^-760 in the FP64 form");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
     return fail(SW2D_E_DEVICE, "no HIP device available");

@@ -155,6 +155,28 @@ def test_library_friction_pow_is_the_oracle_sequence():
         assert sw2d.friction_pow(x) == O.pow_fric(x), x
 
 
+def test_fp64_form_refuses_g_th_below_its_square_root_range():
+    """The FP64 step takes square roots of g h for h > th with a sequence
+    valid from 2^-767 on: sw2d_create refuses g th < 2^-760 (before it looks
+    for a device), and accepts the reference's th = 1e-4 down to g th =
+    2^-760 as far as the argument checks go."""
+    z = np.zeros((8, 8))
+    for th in (0.0, 1e-300):
+        with pytest.raises(sw2d.SW2DError) as e:
+            sw2d.Sim(z, z, 1.0, precision=sw2d.FP64, th=th)
+        assert e.value.code == sw2d.E_INVALID and "th" in str(e.value)
+    try:
+        import torch
+        has_gpu = torch.cuda.is_available()
+    except ImportError:
+        has_gpu = False
+    if not has_gpu:
+        for th in (1e-4, 2.0 ** -760 / 9.80 * 1.0000001):
+            with pytest.raises(sw2d.SW2DError) as e:
+                sw2d.Sim(z, z, 1.0, precision=sw2d.FP64, th=th)
+            assert e.value.code == sw2d.E_DEVICE
+
+
 def test_fp32_form_refuses_a_threshold_below_a_normal_float():
     """sw2d_create checks its arguments before it looks for a device: the
     FP32 form refuses th < 1e-30 (its friction power needs face depths above
